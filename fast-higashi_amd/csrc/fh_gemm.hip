@@ -1,0 +1,485 @@
+// Grouped / batched GEMM on gfx950 matrix cores.
+//
+//  * fp32 and bf16 operands, fp32 accumulate:  v_mfma_f32_16x16x4_f32 / v_mfma_f32_16x16x32_bf16
+//  * fp8 (e4m3) operands with per-batch dequant scales:  v_mfma_f32_16x16x32_fp8_fp8
+//  * fp64:  v_mfma_f64_16x16x4_f64 (Gram matrices and the polar iteration)
+//
+// One launch covers every descriptor of a group table: the workgroup's linear id is
+// mapped to (descriptor, batch, k-split, m-tile, n-tile) so that all chromosomes,
+// bin tiles and cell chunks of a sweep pass go out in a single grid.
+//
+// LDS tiles are always stored k-contiguous ([row][k], 128 data bytes + 16 pad per row)
+// so every MFMA operand fetch is two ds_read_b128.  Operands whose contiguous
+// dimension is M (or N) are transposed in registers on the way into LDS.
+#include "fh_common.h"
+#include "fh_gemm.h"
+
+namespace {
+
+constexpr int NT = 256;        // threads per workgroup (4 waves)
+constexpr int LDS_ROW = 36;    // u32 per LDS row: 32 data + 4 pad (144 B, conflict-free b128 reads)
+
+template <typename T> struct Elem;
+template <> struct Elem<float>   { static constexpr int VEC = 4;  static constexpr int BK = 32;  };
+template <> struct Elem<__bf16>  { static constexpr int VEC = 8;  static constexpr int BK = 64;  };
+template <> struct Elem<uint8_t> { static constexpr int VEC = 16; static constexpr int BK = 128; };
+
+__device__ __forceinline__ int find_desc(const FhGemmDesc* d, int n, int64_t t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (d[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// zero the elements of a 16-byte vector whose index (first + e) >= limit
+template <typename T>
+__device__ __forceinline__ u32x4 mask_tail(u32x4 v, int64_t first, int64_t limit) {
+  constexpr int VEC = Elem<T>::VEC;
+  if (first + VEC <= limit) return v;
+  union { u32x4 v; T e[VEC]; uint8_t b[16]; } u;
+  u.v = v;
+  constexpr int EB = 16 / VEC;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e)
+    if (first + e >= limit)
+#pragma unroll
+      for (int q = 0; q < EB; ++q) u.b[e * EB + q] = 0;
+  return u.v;
+}
+
+// Operand tile whose contiguous dimension is k: ROWS rows x 128 bytes.
+template <typename T, int ROWS>
+struct KMajor {
+  static constexpr int VEC = Elem<T>::VEC;
+  static constexpr int UNITS = ROWS * 8;
+  static constexpr int PER = (UNITS + NT - 1) / NT;
+  u32x4 r[PER];
+  __device__ __forceinline__ void load(const T* base, int64_t s_row, int64_t rows_valid,
+                                       int64_t k0, int64_t K, int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int u = tid + p * NT;
+      int row = u >> 3, v = u & 7;
+      int64_t kk = k0 + (int64_t)v * VEC;
+      u32x4 x = {0u, 0u, 0u, 0u};
+      if (u < UNITS && row < rows_valid && kk < K) {
+        x = __builtin_nontemporal_load((const u32x4*)(base + (int64_t)row * s_row + kk));
+        x = mask_tail<T>(x, kk, K);
+      }
+      r[p] = x;
+    }
+  }
+  __device__ __forceinline__ void store(uint32_t* lds, int tid) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int u = tid + p * NT;
+      if (u < UNITS) {
+        int row = u >> 3, v = u & 7;
+        *(u32x4*)(lds + row * LDS_ROW + v * 4) = r[p];
+      }
+    }
+  }
+};
+
+// Operand tile whose contiguous dimension is the row (m or n): units of 4 k x VEC rows.
+template <typename T, int ROWS>
+struct MNMajor {
+  static constexpr int VEC = Elem<T>::VEC;
+  static constexpr int BK = Elem<T>::BK;
+  static_assert(ROWS % VEC == 0, "rows must be a multiple of the vector width");
+  static constexpr int MB = ROWS / VEC;
+  static constexpr int UNITS = (BK / 4) * MB;
+  static constexpr int PER = (UNITS + NT - 1) / NT;
+  u32x4 r[PER][4];
+  __device__ __forceinline__ void load(const T* base, int64_t s_k, int64_t rows_valid,
+                                       int64_t k0, int64_t K, int tid) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int u = tid + p * NT;
+      int mb = u % MB, kb = u / MB;
+      int64_t m = (int64_t)mb * VEC;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int64_t kj = k0 + kb * 4 + j;
+        u32x4 x = {0u, 0u, 0u, 0u};
+        if (u < UNITS && kj < K && m < rows_valid) {
+          x = __builtin_nontemporal_load((const u32x4*)(base + kj * s_k + m));
+          x = mask_tail<T>(x, m, rows_valid);
+        }
+        r[p][j] = x;
+      }
+    }
+  }
+  __device__ __forceinline__ void store(uint32_t* lds, int tid) const {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      int u = tid + p * NT;
+      if (u >= UNITS) continue;
+      int mb = u % MB, kb = u / MB;
+      if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          u32x4 o = {r[p][0][e], r[p][1][e], r[p][2][e], r[p][3][e]};
+          *(u32x4*)(lds + (mb * 4 + e) * LDS_ROW + kb * 4) = o;
+        }
+      } else if constexpr (sizeof(T) == 2) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          int w = e >> 1, sh = (e & 1) * 16;
+          uint32_t h0 = (r[p][0][w] >> sh) & 0xffffu, h1 = (r[p][1][w] >> sh) & 0xffffu;
+          uint32_t h2 = (r[p][2][w] >> sh) & 0xffffu, h3 = (r[p][3][w] >> sh) & 0xffffu;
+          u32x2 o = {h0 | (h1 << 16), h2 | (h3 << 16)};
+          *(u32x2*)(lds + (mb * 8 + e) * LDS_ROW + kb * 2) = o;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          int w = e >> 2, sh = (e & 3) * 8;
+          uint32_t b0 = (r[p][0][w] >> sh) & 0xffu, b1 = (r[p][1][w] >> sh) & 0xffu;
+          uint32_t b2 = (r[p][2][w] >> sh) & 0xffu, b3 = (r[p][3][w] >> sh) & 0xffu;
+          lds[(mb * 16 + e) * LDS_ROW + kb] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+        }
+      }
+    }
+  }
+};
+
+template <typename T, bool KMAJ, int ROWS>
+using Loader = typename std::conditional<KMAJ, KMajor<T, ROWS>, MNMajor<T, ROWS>>::type;
+
+template <typename T>
+__device__ __forceinline__ void mfma_step(const u32x4 (&a)[2], const u32x4 (&b)[2], floatx4& acc) {
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a[t >> 2][t & 3]),
+                                                 __uint_as_float(b[t >> 2][t & 3]), acc, 0, 0, 0);
+  } else if constexpr (sizeof(T) == 2) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[t]),
+                                                    __builtin_bit_cast(bf16x8, b[t]), acc, 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      long av = (long)a[t >> 1][(t & 1) * 2] | ((long)a[t >> 1][(t & 1) * 2 + 1] << 32);
+      long bv = (long)b[t >> 1][(t & 1) * 2] | ((long)b[t >> 1][(t & 1) * 2 + 1] << 32);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, bv, acc, 0, 0, 0);
+    }
+  }
+}
+
+// WM_T x WN_T MFMA tiles (16x16) per wave, 2 x 2 waves per workgroup.
+template <typename T, bool AK, bool BK_, int WM_T, int WN_T>
+__global__ __launch_bounds__(NT) void gemm_mfma_kernel(const FhGemmDesc* __restrict__ descs,
+                                                       int n_desc, FhGemmDesc single) {
+  constexpr int BM = 32 * WM_T, BN = 32 * WN_T, BK = Elem<T>::BK;
+  __shared__ uint32_t As[BM * LDS_ROW];
+  __shared__ uint32_t Bs[BN * LDS_ROW];
+
+  const int64_t t = blockIdx.x;
+  const FhGemmDesc* dp = &single;
+  if (descs != nullptr) dp = &descs[find_desc(descs, n_desc, t)];
+  const FhGemmDesc d = *dp;
+  if (d.skip && *d.skip) return;
+  const int ksplit = d.flags >> 8 ? (d.flags >> 8) : 1;
+  int64_t local = t - d.tile_begin;
+  const int64_t per_batch = (int64_t)d.tiles_m * d.tiles_n * ksplit;
+  if (local >= per_batch * d.batch) return;
+  const int64_t b = local / per_batch;
+  local -= b * per_batch;
+  const int ks = (int)(local % ksplit);
+  local /= ksplit;
+  const int64_t tm = local / d.tiles_n, tn = local % d.tiles_n;
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+
+  // k range of this split (multiple of BK)
+  const int64_t kchunk = ((d.K + ksplit - 1) / ksplit + BK - 1) / BK * BK;
+  const int64_t kbeg = ks * kchunk;
+  const int64_t kend = kbeg + kchunk < d.K ? kbeg + kchunk : d.K;
+
+  const T* A = (const T*)d.A + b * d.sAb;
+  const T* B = (const T*)d.B + b * d.sBb;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  Loader<T, AK, BM> la;
+  Loader<T, BK_, BN> lb;
+  const T* Abase = AK ? A + m0 * d.sAm : A + m0 * d.sAm;
+  const T* Bbase = BK_ ? B + n0 * d.sBn : B + n0 * d.sBn;
+  const int64_t a_s = AK ? d.sAm : d.sAk;
+  const int64_t b_s = BK_ ? d.sBn : d.sBk;
+  const int64_t rows_a = d.M - m0, rows_b = d.N - n0;
+
+  floatx4 acc[WM_T][WN_T];
+#pragma unroll
+  for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+    for (int j = 0; j < WN_T; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  if (kbeg < kend) {
+    la.load(Abase, a_s, rows_a, kbeg, kend, tid);
+    lb.load(Bbase, b_s, rows_b, kbeg, kend, tid);
+  }
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();
+    la.store(As, tid);
+    lb.store(Bs, tid);
+    __syncthreads();
+    if (k0 + BK < kend) {
+      la.load(Abase, a_s, rows_a, k0 + BK, kend, tid);
+      lb.load(Bbase, b_s, rows_b, k0 + BK, kend, tid);
+    }
+    u32x4 af[WM_T][2], bf[WN_T][2];
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i) {
+      const uint32_t* p = As + (wm * WM_T * 16 + i * 16 + (lane & 15)) * LDS_ROW + (lane >> 4) * 8;
+      af[i][0] = *(const u32x4*)p;
+      af[i][1] = *(const u32x4*)(p + 4);
+    }
+#pragma unroll
+    for (int j = 0; j < WN_T; ++j) {
+      const uint32_t* p = Bs + (wn * WN_T * 16 + j * 16 + (lane & 15)) * LDS_ROW + (lane >> 4) * 8;
+      bf[j][0] = *(const u32x4*)p;
+      bf[j][1] = *(const u32x4*)(p + 4);
+    }
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+      for (int j = 0; j < WN_T; ++j) mfma_step<T>(af[i], bf[j], acc[i][j]);
+  }
+
+  float scale = (float)d.alpha;
+  if (d.scale_a) scale *= d.scale_a[b];
+  if (d.scale_b) scale *= d.scale_b[b];
+  float* C = (float*)d.C + b * d.sCb;
+  uint16_t* Ch = (uint16_t*)d.C + b * d.sCb;
+  const bool atomic = (d.flags & FH_GEMM_ATOMIC) != 0;
+  const bool out_bf16 = (d.flags & FH_GEMM_OUT_BF16) != 0;
+  const float beta = (float)d.beta, diag = (float)d.diag;
+#pragma unroll
+  for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+    for (int j = 0; j < WN_T; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int64_t row = m0 + wm * WM_T * 16 + i * 16 + (lane >> 4) * 4 + r;
+        int64_t col = n0 + wn * WN_T * 16 + j * 16 + (lane & 15);
+        if (row < d.M && col < d.N) {
+          float* cp = C + row * d.sCm + col * d.sCn;
+          float v = scale * acc[i][j][r];
+          if (out_bf16) {
+            if (row == col) v += diag;
+            Ch[row * d.sCm + col * d.sCn] = fh_f32_to_bf16_bits(v);
+          } else if (atomic) {
+            if (ks == 0 && row == col) v += diag;
+            atomicAdd(cp, v);
+          } else {
+            if (beta != 0.f) v += beta * *cp;
+            if (row == col) v += diag;
+            *cp = v;
+          }
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// fp64: 64 x 64 x 16 tiles, 4 waves each 32 x 32 (2 x 2 MFMA f64 16x16x4)
+// A, B may be fp32 (converted exactly) or fp64; C fp64 or fp32.
+// ---------------------------------------------------------------------------
+template <typename TA, typename TB, typename TC>
+__global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restrict__ descs,
+                                                      int n_desc, FhGemmDesc single) {
+  constexpr int BM = 64, BN = 64, BKD = 16, LR = BKD + 1;
+  __shared__ double As[BM * LR];
+  __shared__ double Bs[BN * LR];
+  const int64_t t = blockIdx.x;
+  const FhGemmDesc* dp = &single;
+  if (descs != nullptr) dp = &descs[find_desc(descs, n_desc, t)];
+  const FhGemmDesc d = *dp;
+  if (d.skip && *d.skip) return;
+  const int ksplit = d.flags >> 8 ? (d.flags >> 8) : 1;
+  int64_t local = t - d.tile_begin;
+  const int64_t per_batch = (int64_t)d.tiles_m * d.tiles_n * ksplit;
+  if (local >= per_batch * d.batch) return;
+  const int64_t b = local / per_batch;
+  local -= b * per_batch;
+  const int ks = (int)(local % ksplit);
+  local /= ksplit;
+  const int64_t tm = local / d.tiles_n, tn = local % d.tiles_n;
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const int64_t kchunk = ((d.K + ksplit - 1) / ksplit + BKD - 1) / BKD * BKD;
+  const int64_t kbeg = ks * kchunk;
+  const int64_t kend = kbeg + kchunk < d.K ? kbeg + kchunk : d.K;
+
+  const TA* A = (const TA*)d.A + b * d.sAb;
+  const TB* B = (const TB*)d.B + b * d.sBb;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const bool a_k_fast = d.sAk == 1, b_k_fast = d.sBk == 1;
+
+  doublex4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = doublex4{0., 0., 0., 0.};
+
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BKD) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      int e = tid + p * NT;
+      int m, k;
+      if (a_k_fast) { m = e >> 4; k = e & 15; } else { m = e & 63; k = e >> 6; }
+      double v = 0.;
+      if (m0 + m < d.M && k0 + k < kend) v = (double)A[(m0 + m) * d.sAm + (k0 + k) * d.sAk];
+      As[m * LR + k] = v;
+      int n;
+      if (b_k_fast) { n = e >> 4; k = e & 15; } else { n = e & 63; k = e >> 6; }
+      double w = 0.;
+      if (n0 + n < d.N && k0 + k < kend) w = (double)B[(k0 + k) * d.sBk + (n0 + n) * d.sBn];
+      Bs[n * LR + k] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BKD; kk += 4) {
+      double a[2], bb[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[(wm * 32 + i * 16 + (lane & 15)) * LR + kk + (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bb[j] = Bs[(wn * 32 + j * 16 + (lane & 15)) * LR + kk + (lane >> 4)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
+    }
+  }
+
+  TC* C = (TC*)d.C + b * d.sCb;
+  const bool atomic = (d.flags & FH_GEMM_ATOMIC) != 0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int64_t row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;  // f64 MFMA: item r of lane l is row 4r + l/16
+        int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (row < d.M && col < d.N) {
+          TC* cp = C + row * d.sCm + col * d.sCn;
+          double v = d.alpha * acc[i][j][r];
+          if (atomic) {
+            if (ks == 0 && row == col) v += d.diag;
+            atomicAdd(cp, (TC)v);
+          } else {
+            if (d.beta != 0.) v += d.beta * (double)*cp;
+            if (row == col) v += d.diag;
+            *cp = (TC)v;
+          }
+        }
+      }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+// Fill tile_begin / tiles_m / tiles_n of each descriptor for a given tile shape;
+// returns the total workgroup count.
+extern "C" int64_t fh_gemm_prepare(FhGemmDesc* descs, int n, int bm, int bn) {
+  int64_t total = 0;
+  for (int i = 0; i < n; ++i) {
+    FhGemmDesc& d = descs[i];
+    int ks = (d.flags >> 8) ? (d.flags >> 8) : 1;
+    d.tiles_m = (int32_t)fh_cdiv(d.M, bm);
+    d.tiles_n = (int32_t)fh_cdiv(d.N, bn);
+    d.tile_begin = total;
+    total += (int64_t)d.tiles_m * d.tiles_n * ks * d.batch;
+  }
+  return total;
+}
+
+namespace {
+
+template <typename T, int WM, int WN>
+int launch_typed(bool ak, bool bk, const FhGemmDesc* dd, int n, const FhGemmDesc& single,
+                 int64_t tiles, hipStream_t st) {
+  dim3 grid((unsigned)tiles), block(NT);
+  if (ak && bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, true, true, WM, WN>), grid, block, 0, st, dd, n, single);
+  else if (ak && !bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, true, false, WM, WN>), grid, block, 0, st, dd, n, single);
+  else if (!ak && bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, false, true, WM, WN>), grid, block, 0, st, dd, n, single);
+  else hipLaunchKernelGGL((gemm_mfma_kernel<T, false, false, WM, WN>), grid, block, 0, st, dd, n, single);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+template <typename T>
+int launch_shape(int shape, bool ak, bool bk, const FhGemmDesc* dd, int n, const FhGemmDesc& single,
+                 int64_t tiles, hipStream_t st) {
+  switch (shape) {
+    case 0: return launch_typed<T, 4, 4>(ak, bk, dd, n, single, tiles, st);   // 128 x 128
+    case 1: return launch_typed<T, 4, 2>(ak, bk, dd, n, single, tiles, st);   // 128 x 64
+    case 2: return launch_typed<T, 2, 2>(ak, bk, dd, n, single, tiles, st);   //  64 x 64
+    case 3: return launch_typed<T, 4, 5>(ak, bk, dd, n, single, tiles, st);   // 128 x 160
+    case 4: return launch_typed<T, 2, 4>(ak, bk, dd, n, single, tiles, st);   //  64 x 128
+    case 5: return launch_typed<T, 8, 4>(ak, bk, dd, n, single, tiles, st);   // 256 x 128
+    default: fh_set_error("unknown gemm tile shape %d", shape); return FH_ERR_ARG;
+  }
+}
+
+}  // namespace
+
+extern "C" void fh_gemm_shape_dims(int shape, int* bm, int* bn) {
+  static const int dims[6][2] = {{128, 128}, {128, 64}, {64, 64}, {128, 160}, {64, 128}, {256, 128}};
+  if (shape < 0 || shape > 5) { *bm = *bn = 0; return; }
+  *bm = dims[shape][0];
+  *bn = dims[shape][1];
+}
+
+// descs_dev: device table (prepared with fh_gemm_prepare for this shape), or null to use *single.
+// ak / bk: 1 if A (B) is k-contiguous, 0 if m (n)-contiguous.
+extern "C" int fh_gemm_launch(int dtype, int shape, int ak, int bk, const FhGemmDesc* descs_dev,
+                              int n_desc, const FhGemmDesc* single, int64_t total_tiles,
+                              hipStream_t stream) {
+  if (total_tiles <= 0) return FH_OK;
+  FhGemmDesc s1{};
+  if (single) s1 = *single;
+  FH_REQUIRE(descs_dev != nullptr || single != nullptr, "gemm: no descriptor");
+  FH_REQUIRE(total_tiles < (int64_t)1 << 31, "gemm: too many tiles");
+  switch (dtype) {
+    case FH_F32: return launch_shape<float>(shape, ak, bk, descs_dev, n_desc, s1, total_tiles, stream);
+    case FH_BF16: return launch_shape<__bf16>(shape, ak, bk, descs_dev, n_desc, s1, total_tiles, stream);
+    case FH_FP8: return launch_shape<uint8_t>(shape, ak, bk, descs_dev, n_desc, s1, total_tiles, stream);
+    default: fh_set_error("gemm: unsupported dtype %d", dtype); return FH_ERR_UNSUPPORTED;
+  }
+}
+
+// fp64 family: a_dtype/b_dtype in {FH_F32, FH_F64}, c_dtype in {FH_F32, FH_F64}; tiles 64 x 64.
+extern "C" int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhGemmDesc* descs_dev,
+                                int n_desc, const FhGemmDesc* single, int64_t total_tiles,
+                                hipStream_t stream) {
+  if (total_tiles <= 0) return FH_OK;
+  FhGemmDesc s1{};
+  if (single) s1 = *single;
+  FH_REQUIRE(descs_dev != nullptr || single != nullptr, "gemm64: no descriptor");
+  dim3 grid((unsigned)total_tiles), block(NT);
+  int key = (a_dtype == FH_F64) * 4 + (b_dtype == FH_F64) * 2 + (c_dtype == FH_F64);
+  switch (key) {
+    case 0: hipLaunchKernelGGL((gemm_f64_kernel<float, float, float>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+    case 1: hipLaunchKernelGGL((gemm_f64_kernel<float, float, double>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+    case 2: hipLaunchKernelGGL((gemm_f64_kernel<float, double, float>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+    case 3: hipLaunchKernelGGL((gemm_f64_kernel<float, double, double>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+    case 4: hipLaunchKernelGGL((gemm_f64_kernel<double, float, float>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+    case 5: hipLaunchKernelGGL((gemm_f64_kernel<double, float, double>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+    case 6: hipLaunchKernelGGL((gemm_f64_kernel<double, double, float>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+    default: hipLaunchKernelGGL((gemm_f64_kernel<double, double, double>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+  }
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}

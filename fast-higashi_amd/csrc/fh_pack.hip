@@ -1,0 +1,247 @@
+// Packing of scHi-C COO contacts into dense banded tiles.
+//
+//  * stable counting sort by row -> CSR order identical to scipy coo_tocsr
+//    (sparse_for_schic.py:153-182): chunk histograms, per-row scan over chunks, then
+//    an in-order placement pass per chunk in which lanes of one wave that share a row
+//    are ranked with ballots.
+//  * band scatter: entry (row, col, cell) -> dense tile element; the tile buffer is
+//    pre-filled with the 1e-8 floor of densify_jit (sparse_for_schic.py:320).
+//  * tiled transpose (cell-major staging -> cell-innermost X layout) with dtype change.
+#include "fh_common.h"
+
+namespace {
+constexpr int NT = 256;
+
+__global__ void chunk_hist_kernel(const int32_t* __restrict__ row, int64_t nnz, int64_t chunk, int32_t n_rows,
+                                  int32_t* __restrict__ hist) {
+  extern __shared__ int32_t h[];
+  for (int i = threadIdx.x; i < n_rows; i += NT) h[i] = 0;
+  __syncthreads();
+  const int64_t b = blockIdx.x;
+  const int64_t e0 = b * chunk, e1 = e0 + chunk < nnz ? e0 + chunk : nnz;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += NT) atomicAdd(&h[row[e]], 1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_rows; i += NT) hist[b * n_rows + i] = h[i];
+}
+
+// per row: exclusive scan over chunks (in place), totals out
+__global__ void chunk_scan_kernel(int32_t* __restrict__ hist, int64_t n_chunks, int32_t n_rows,
+                                  int64_t* __restrict__ total) {
+  const int64_t r = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (r >= n_rows) return;
+  int64_t run = 0;
+  for (int64_t c = 0; c < n_chunks; ++c) {
+    int32_t v = hist[c * n_rows + r];
+    hist[c * n_rows + r] = (int32_t)run;
+    run += v;
+  }
+  total[r] = run;
+}
+
+// indptr = exclusive scan of totals (single block, sequential blocks of 256)
+__global__ void indptr_kernel(const int64_t* __restrict__ total, int32_t n_rows, int64_t* __restrict__ indptr) {
+  __shared__ int64_t buf[NT];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int32_t base = 0; base < n_rows; base += NT) {
+    int32_t i = base + threadIdx.x;
+    int64_t v = i < n_rows ? total[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < NT; off <<= 1) {
+      int64_t t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n_rows) indptr[i] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == NT - 1) carry += buf[NT - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) indptr[n_rows] = carry;
+}
+
+// one wave per chunk: place elements in input order
+__global__ __launch_bounds__(64) void chunk_place_kernel(const int32_t* __restrict__ row, int64_t nnz, int64_t chunk,
+                                                         int32_t n_rows, const int32_t* __restrict__ hist,
+                                                         const int64_t* __restrict__ indptr,
+                                                         int64_t* __restrict__ perm) {
+  extern __shared__ int64_t cnt[];
+  const int64_t b = blockIdx.x;
+  const int lane = threadIdx.x;
+  for (int i = lane; i < n_rows; i += 64) cnt[i] = indptr[i] + hist[b * n_rows + i];
+  __syncthreads();
+  const int64_t e0 = b * chunk, e1 = e0 + chunk < nnz ? e0 + chunk : nnz;
+  for (int64_t base = e0; base < e1; base += 64) {
+    const int64_t e = base + lane;
+    const bool valid = e < e1;
+    const int32_t r = valid ? row[e] : -1;
+    uint64_t active = __ballot(valid);
+    while (active) {
+      const int leader = __ffsll((unsigned long long)active) - 1;
+      const int32_t rl = __shfl(r, leader);
+      const uint64_t match = __ballot(valid && r == rl);
+      if (valid && r == rl) {
+        const uint64_t lower = match & ((1ull << lane) - 1ull);
+        const int64_t pos = cnt[rl] + __popcll(lower);
+        perm[pos] = e;
+      }
+      __syncthreads();
+      if (lane == leader) cnt[rl] += __popcll(match);
+      __syncthreads();
+      active &= ~match;
+    }
+  }
+}
+
+// out[k][i] = in[k][perm[i]] for the three int32 index rows, values likewise
+__global__ void gather_coo_kernel(const int32_t* __restrict__ in_idx, const float* __restrict__ in_val,
+                                  const int64_t* __restrict__ perm, int64_t nnz, int32_t* __restrict__ out_idx,
+                                  float* __restrict__ out_val) {
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * NT) {
+    const int64_t p = perm[i];
+    out_idx[i] = in_idx[p];
+    out_idx[nnz + i] = in_idx[nnz + p];
+    out_idx[2 * nnz + i] = in_idx[2 * nnz + p];
+    out_val[i] = in_val[p];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct FhBandTile {
+  int64_t offset;      // element offset of the tile in the output buffer
+  int64_t s_row;       // stride of a local row
+  int64_t s_col;       // stride of a local column
+  int64_t s_cell;      // stride of a cell
+  int32_t row_start;   // first bin of the tile
+  int32_t rows;        // bins in the tile
+  int32_t shift;       // absolute column -> local column offset
+  int32_t width;       // local columns
+} FhBandTile;
+
+}
+
+namespace {
+
+// COO entries of one chromosome (rows sorted or not) -> tiles; tile_of_row maps a row to its tile.
+__global__ void band_scatter_kernel(const int32_t* __restrict__ idx, const float* __restrict__ val, int64_t nnz,
+                                    const int32_t* __restrict__ tile_of_row, const FhBandTile* __restrict__ tiles,
+                                    int32_t cell_lo, int32_t cell_hi, void* out, int out_dt,
+                                    unsigned long long* __restrict__ dropped) {
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * NT) {
+    const int32_t r = idx[i], c = idx[nnz + i], k = idx[2 * nnz + i];
+    if (k < cell_lo || k >= cell_hi) continue;
+    const FhBandTile t = tiles[tile_of_row[r]];
+    const int32_t lr = r - t.row_start, lc = c - t.shift;
+    if (lr < 0 || lr >= t.rows || lc < 0 || lc >= t.width) {
+      if (dropped) atomicAdd(dropped, 1ull);
+      continue;
+    }
+    const int64_t o = t.offset + lr * t.s_row + lc * t.s_col + (int64_t)(k - cell_lo) * t.s_cell;
+    if (out_dt == FH_BF16) ((uint16_t*)out)[o] = fh_f32_to_bf16_bits(val[i]);
+    else ((float*)out)[o] = val[i];
+  }
+}
+
+// dst[p * ldd + c] = src[c * lds + p]  for c < ncols (cells), p < nrows; 64 x 64 LDS tiles
+__global__ void transpose_kernel(const float* __restrict__ src, int64_t lds_, int64_t ncells, int64_t npos,
+                                 void* dst, int ddt, int64_t ldd, const int64_t* __restrict__ pos_map) {
+  __shared__ float tile[64][65];
+  const int64_t c0 = (int64_t)blockIdx.y * 64, p0 = (int64_t)blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int i = ty; i < 64; i += 4) {
+    int64_t c = c0 + i, p = p0 + tx;
+    tile[i][tx] = (c < ncells && p < npos) ? src[c * lds_ + p] : 0.f;
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) {
+    int64_t p = p0 + i, c = c0 + tx;
+    if (p < npos && c < ncells) {
+      int64_t drow = pos_map ? pos_map[p] : p;
+      if (drow < 0) continue;
+      if (ddt == FH_BF16) ((uint16_t*)dst)[drow * ldd + c] = fh_f32_to_bf16_bits(tile[tx][i]);
+      else ((float*)dst)[drow * ldd + c] = tile[tx][i];
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Workspace sizes for fh_csr_sort (bytes).
+int64_t fh_csr_sort_workspace(int64_t nnz, int32_t n_rows, int64_t chunk) {
+  int64_t n_chunks = nnz > 0 ? (nnz + chunk - 1) / chunk : 1;
+  return n_chunks * n_rows * 4 + (int64_t)(n_rows + 1) * 8 + nnz * 8 + 256;
+}
+
+// Stable sort of COO (3 x nnz int32 indices, row first; nnz float values) by row.
+// Outputs sorted indices/values and indptr (n_rows + 1, int64).
+int fh_csr_sort(const int32_t* idx, const float* val, int64_t nnz, int32_t n_rows, int64_t chunk,
+                int32_t* out_idx, float* out_val, int64_t* indptr, void* workspace, int64_t ws_bytes,
+                hipStream_t st) {
+  FH_REQUIRE(n_rows > 0 && n_rows <= 16384, "csr_sort: n_rows=%d out of range", n_rows);
+  FH_REQUIRE(chunk >= 64 && chunk % 64 == 0, "csr_sort: chunk must be a multiple of 64");
+  FH_REQUIRE(ws_bytes >= fh_csr_sort_workspace(nnz, n_rows, chunk), "csr_sort: workspace too small");
+  const int64_t n_chunks = nnz > 0 ? (nnz + chunk - 1) / chunk : 0;
+  char* ws = (char*)workspace;
+  int32_t* hist = (int32_t*)ws;
+  int64_t* total = (int64_t*)(ws + ((n_chunks > 0 ? n_chunks : 1) * n_rows * 4 + 7) / 8 * 8);
+  int64_t* perm = total + (n_rows + 1);
+  if (n_chunks > 0) {
+    hipLaunchKernelGGL(chunk_hist_kernel, dim3((unsigned)n_chunks), dim3(NT), n_rows * 4, st, idx, nnz, chunk,
+                       n_rows, hist);
+    FH_CHECK_LAUNCH();
+    hipLaunchKernelGGL(chunk_scan_kernel, dim3((n_rows + NT - 1) / NT), dim3(NT), 0, st, hist, n_chunks, n_rows,
+                       total);
+    FH_CHECK_LAUNCH();
+  } else {
+    hipLaunchKernelGGL(chunk_scan_kernel, dim3((n_rows + NT - 1) / NT), dim3(NT), 0, st, hist, (int64_t)0,
+                       n_rows, total);
+    FH_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(indptr_kernel, dim3(1), dim3(NT), 0, st, total, n_rows, indptr);
+  FH_CHECK_LAUNCH();
+  if (n_chunks > 0) {
+    hipLaunchKernelGGL(chunk_place_kernel, dim3((unsigned)n_chunks), dim3(64), n_rows * 8, st, idx, nnz, chunk,
+                       n_rows, hist, indptr, perm);
+    FH_CHECK_LAUNCH();
+    unsigned g = (unsigned)((nnz + NT - 1) / NT);
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(gather_coo_kernel, dim3(g), dim3(NT), 0, st, idx, val, perm, nnz, out_idx, out_val);
+    FH_CHECK_LAUNCH();
+  }
+  return FH_OK;
+}
+
+// Scatter COO entries of one chromosome into dense band tiles (see FhBandTile).
+int fh_band_scatter(const int32_t* idx, const float* val, int64_t nnz, const int32_t* tile_of_row,
+                    const FhBandTile* tiles_dev, int32_t cell_lo, int32_t cell_hi, void* out, int out_dtype,
+                    unsigned long long* dropped_dev, hipStream_t st) {
+  if (nnz <= 0) return FH_OK;
+  FH_REQUIRE(out_dtype == FH_F32 || out_dtype == FH_BF16, "band_scatter: dtype");
+  unsigned g = (unsigned)((nnz + NT - 1) / NT);
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(band_scatter_kernel, dim3(g), dim3(NT), 0, st, idx, val, nnz, tile_of_row, tiles_dev, cell_lo,
+                     cell_hi, out, out_dtype, dropped_dev);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+// dst[pos_map[p] * ldd + c] = src[c * lds + p]   (pos_map null: identity; negative entries skipped)
+int fh_transpose(const float* src, int64_t lds_, int64_t ncells, int64_t npos, void* dst, int dst_dtype,
+                 int64_t ldd, const int64_t* pos_map, hipStream_t st) {
+  if (ncells <= 0 || npos <= 0) return FH_OK;
+  FH_REQUIRE(fh_cdiv(ncells, 64) < 65536, "transpose: too many cells");
+  dim3 grid((unsigned)fh_cdiv(npos, 64), (unsigned)fh_cdiv(ncells, 64));
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(NT), 0, st, src, lds_, ncells, npos, dst, dst_dtype, ldd, pos_map);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,340 @@
+// Element-wise and reduction kernels around the sweep's GEMMs.
+// All reductions are deterministic (fixed partition, fixed order): repeated runs on
+// the same inputs give bit-identical results.
+#include "fh_common.h"
+
+namespace {
+constexpr int NT = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t += red[w];
+  return t;
+}
+
+template <typename T> __device__ __forceinline__ double ld_as_f64(const void* p, int64_t i) {
+  return (double)((const T*)p)[i];
+}
+__device__ __forceinline__ double load_any(const void* p, int dt, int64_t i) {
+  switch (dt) {
+    case FH_F32: return ((const float*)p)[i];
+    case FH_F64: return ((const double*)p)[i];
+    case FH_BF16: return (double)(float)((const __bf16*)p)[i];
+    default: return 0.;
+  }
+}
+__device__ __forceinline__ void store_any(void* p, int dt, int64_t i, double v) {
+  switch (dt) {
+    case FH_F32: ((float*)p)[i] = (float)v; break;
+    case FH_F64: ((double*)p)[i] = v; break;
+    case FH_BF16: ((uint16_t*)p)[i] = fh_f32_to_bf16_bits((float)v); break;
+    default: break;
+  }
+}
+
+__global__ void fill_kernel(void* p, int esize, int64_t n, uint64_t pattern) {
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    if (esize == 4) ((uint32_t*)p)[i] = (uint32_t)pattern;
+    else if (esize == 2) ((uint16_t*)p)[i] = (uint16_t)pattern;
+    else if (esize == 8) ((uint64_t*)p)[i] = pattern;
+    else ((uint8_t*)p)[i] = (uint8_t)pattern;
+  }
+}
+
+// dst[r*dld + c] = src[r*srs + c*scs]   (dtype conversion included)
+__global__ void copy2d_kernel(const void* src, int sdt, int64_t srs, int64_t scs, void* dst, int ddt,
+                              int64_t dld, int64_t rows, int64_t cols, double alpha) {
+  const int64_t n = rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    int64_t r = i / cols, c = i % cols;
+    store_any(dst, ddt, r * dld + c, alpha * load_any(src, sdt, r * srs + c * scs));
+  }
+}
+
+// Row scaling of per-bin blocks:  Y[r][j] *= A[bin0 + r / w][j]   (j < s), one tile per blockIdx.y.
+struct RowScale { float* Y; const float* A; int64_t nrows, w, s, ldy, lda; };
+__global__ void scale_rows_kernel(const RowScale* __restrict__ tab) {
+  const RowScale t = tab[blockIdx.y];
+  const int64_t n = t.nrows * t.s;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    int64_t r = i / t.s, j = i % t.s;
+    t.Y[r * t.ldy + j] *= t.A[(r / t.w) * t.lda + j];
+  }
+}
+
+// Khatri-Rao rows:  L[r][i*sp + j] = A[i][r] * B[j][r]  (j < s; zero for s <= j < sp)
+__global__ void krp_kernel(const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n, int64_t s,
+                           int64_t sp, void* L, int ldt) {
+  const int64_t cols = n * sp, tot = s * cols;
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t)gridDim.x * NT) {
+    int64_t r = e / cols, q = e % cols, i = q / sp, j = q % sp;
+    double v = (j < s) ? (double)A[i * lda + r] * (double)B[j * ldb + r] : 0.;
+    store_any(L, ldt, e, (double)(float)v);
+  }
+}
+
+// W1 rows (i*sp + j), columns r (ld = ldw).
+// mode 0:  out[i][r] = sum_j W1[i*sp+j][r] * B[j][r]      (i < n)
+// mode 1:  out[j][r] = sum_i W1[i*sp+j][r] * A[i][r]      (j < s)
+__global__ void mttkrp_reduce_kernel(const float* W1, int64_t ldw, const float* F, int64_t ldf, int64_t n,
+                                     int64_t s, int64_t sp, int mode, float* out, int64_t ldo) {
+  const int64_t rows = mode == 0 ? n : s;
+  const int64_t tot = rows * s;
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t)gridDim.x * NT) {
+    int64_t a = e / s, r = e % s;
+    double acc = 0.;
+    if (mode == 0) {
+      for (int64_t j = 0; j < s; ++j) acc += (double)W1[(a * sp + j) * ldw + r] * (double)F[j * ldf + r];
+    } else {
+      for (int64_t i = 0; i < n; ++i) acc += (double)W1[(i * sp + a) * ldw + r] * (double)F[i * ldf + r];
+    }
+    out[a * ldo + r] = (float)acc;
+  }
+}
+
+// out = G1 o G2 (o G3)  s x s fp64
+__global__ void hadamard_kernel(const double* G1, const double* G2, const double* G3, int64_t s, int64_t ld,
+                                double* out) {
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < s * s; e += (int64_t)gridDim.x * NT) {
+    int64_t i = e / s, j = e % s;
+    double v = G1[i * ld + j] * G2[i * ld + j];
+    if (G3) v *= G3[i * ld + j];
+    out[i * ld + j] = v;
+  }
+}
+
+// balance_norm (parafac_integrative.py:19-26) on one chromosome, one block per column r:
+// every factor column is unit-normalised, D's column also takes the product of the norms.
+__global__ void balance_kernel(float* A, int64_t lda, int64_t n, float* B, int64_t ldb, float* D, int64_t ldd,
+                               int64_t R, int64_t s, const int* skip) {
+  if (skip && *skip) return;
+  __shared__ double red[NT / 64];
+  const int64_t r = blockIdx.x;
+  double pa = 0., pb = 0., pd = 0.;
+  for (int64_t i = threadIdx.x; i < n; i += NT) { double v = A[i * lda + r]; pa += v * v; }
+  for (int64_t i = threadIdx.x; i < s; i += NT) { double v = B[i * ldb + r]; pb += v * v; }
+  for (int64_t i = threadIdx.x; i < R; i += NT) { double v = D[i * ldd + r]; pd += v * v; }
+  const float na = (float)sqrt(block_sum(pa, red));
+  const float nb = (float)sqrt(block_sum(pb, red));
+  const float nd = (float)sqrt(block_sum(pd, red));
+  const float even = na * nb * nd + 1e-15f;
+  const float ia = 1.f / (na + 1e-15f), ib = 1.f / (nb + 1e-15f), id = even / (nd + 1e-15f);
+  for (int64_t i = threadIdx.x; i < n; i += NT) A[i * lda + r] *= ia;
+  for (int64_t i = threadIdx.x; i < s; i += NT) B[i * ldb + r] *= ib;
+  for (int64_t i = threadIdx.x; i < R; i += NT) D[i * ldd + r] *= id;
+}
+
+// out[0] = sum_{ij} G1 o G2 o G3  (= ||[[A,B,D]]||^2 from the three Grams)
+__global__ void sum3_kernel(const double* G1, const double* G2, const double* G3, int64_t s, int64_t ld,
+                            double* out) {
+  __shared__ double red[NT / 64];
+  double p = 0.;
+  for (int64_t e = threadIdx.x; e < s * s; e += NT) {
+    int64_t i = e / s, j = e % s;
+    p += G1[i * ld + j] * G2[i * ld + j] * G3[i * ld + j];
+  }
+  double t = block_sum(p, red);
+  if (threadIdx.x == 0) *out = t;
+}
+
+// partial sums of squares over rows x cols (ld) blocks: part[blockIdx.x]
+__global__ void sumsq_partial_kernel(const void* X, int dt, int64_t rows, int64_t cols, int64_t ld,
+                                     int64_t rows_per_block, double* part) {
+  __shared__ double red[NT / 64];
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  double p = 0.;
+  for (int64_t r = r0; r < r1; ++r)
+    for (int64_t c = threadIdx.x; c < cols; c += NT) {
+      double v = load_any(X, dt, r * ld + c);
+      p += v * v;
+    }
+  double t = block_sum(p, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// out[g] (+)= sum over segment g of vals (segments given by offsets[g]..offsets[g+1]), fixed order
+__global__ void segment_sum_kernel(const double* vals, const int64_t* offsets, int n_seg, double* out,
+                                   int accumulate) {
+  __shared__ double red[NT / 64];
+  const int g = blockIdx.x;
+  if (g >= n_seg) return;
+  double p = 0.;
+  for (int64_t i = offsets[g] + threadIdx.x; i < offsets[g + 1]; i += NT) p += vals[i];
+  double t = block_sum(p, red);
+  if (threadIdx.x == 0) out[g] = accumulate ? out[g] + t : t;
+}
+
+// <X, Y> over rows x cols with independent strides (fp32/fp64 inputs), written to out[0]
+__global__ void dot2d_kernel(const void* X, int xdt, int64_t ldx, const void* Y, int ydt, int64_t ldy,
+                             int64_t rows, int64_t cols, double* out) {
+  __shared__ double red[NT / 64];
+  double p = 0.;
+  for (int64_t e = threadIdx.x; e < rows * cols; e += NT) {
+    int64_t r = e / cols, c = e % cols;
+    p += load_any(X, xdt, r * ldx + c) * load_any(Y, ydt, r * ldy + c);
+  }
+  double t = block_sum(p, red);
+  if (threadIdx.x == 0) *out = t;
+}
+
+// Sweep error bookkeeping (parafac2_intergrative.py:667-668,710):
+//   err_U[c] = tnorm[c] + core[c] - 2 xU[c],  err_V = sum tnorm + sum core - 2 xV
+//   hist: [rec_error, err_V, xV, then per chrom: rec_block[c], xU[c], core[c]]
+__global__ void sweep_errors_kernel(const double* tnorm, const double* core, const double* xU, const double* xV,
+                                    int K, double* hist) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double st = 0., sc = 0.;
+  for (int c = 0; c < K; ++c) { st += tnorm[c]; sc += core[c]; }
+  double errV = st + sc - 2. * xV[0];
+  hist[0] = sqrt(errV) / sqrt(st);
+  hist[1] = errV;
+  hist[2] = xV[0];
+  for (int c = 0; c < K; ++c) {
+    double eu = tnorm[c] + core[c] - 2. * xU[c];
+    hist[3 + 3 * c] = sqrt(eu) / sqrt(tnorm[c]);
+    hist[4 + 3 * c] = xU[c];
+    hist[5 + 3 * c] = core[c];
+  }
+}
+
+// CP-ALS inner-iteration test (parafac_integrative.py:80-108) from closed forms:
+//   loss = ||T||^2 - 2 <M_D, D> + ||[[A,B,D]]||^2 ; wdiff = (prev - loss) / prev; stop if wdiff < 1e-5
+// state: [0] = previous loss, [1] = have previous; flag: 1 once stopped.
+__global__ void cpals_check_kernel(const double* tnorm2, const double* inner, const double* xhat2, double* state,
+                                   int* flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (*flag) return;
+  double loss = tnorm2[0] - 2. * inner[0] + xhat2[0];
+  if (state[1] != 0.) {
+    double wdiff = (state[0] - loss) / state[0];
+    if (wdiff < 1e-5) *flag = 1;
+  }
+  state[0] = loss;
+  state[1] = 1.;
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct FhRowScale { float* Y; const float* A; int64_t nrows, w, s, ldy, lda; } FhRowScale;
+
+static inline unsigned grid_for(int64_t n) {
+  int64_t g = (n + NT - 1) / NT;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+int fh_fill(void* p, int esize, int64_t n, uint64_t pattern, hipStream_t st) {
+  if (n <= 0) return FH_OK;
+  FH_REQUIRE(esize == 1 || esize == 2 || esize == 4 || esize == 8, "fill: bad element size");
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(NT), 0, st, p, esize, n, pattern);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_copy2d(const void* src, int sdt, int64_t srs, int64_t scs, void* dst, int ddt, int64_t dld,
+              int64_t rows, int64_t cols, double alpha, hipStream_t st) {
+  if (rows <= 0 || cols <= 0) return FH_OK;
+  hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for(rows * cols)), dim3(NT), 0, st, src, sdt, srs, scs, dst, ddt,
+                     dld, rows, cols, alpha);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_scale_rows(const FhRowScale* tab_dev, int n_tiles, int64_t max_elems, hipStream_t st) {
+  if (n_tiles <= 0) return FH_OK;
+  unsigned gx = grid_for(max_elems);
+  if (gx > 4096) gx = 4096;
+  hipLaunchKernelGGL(scale_rows_kernel, dim3(gx, n_tiles), dim3(NT), 0, st, (const RowScale*)tab_dev);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_krp(const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n, int64_t s, int64_t sp,
+           void* L, int l_dtype, hipStream_t st) {
+  hipLaunchKernelGGL(krp_kernel, dim3(grid_for(s * n * sp)), dim3(NT), 0, st, A, lda, B, ldb, n, s, sp, L, l_dtype);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_mttkrp_reduce(const float* W1, int64_t ldw, const float* F, int64_t ldf, int64_t n, int64_t s,
+                     int64_t sp, int mode, float* out, int64_t ldo, hipStream_t st) {
+  int64_t tot = (mode == 0 ? n : s) * s;
+  hipLaunchKernelGGL(mttkrp_reduce_kernel, dim3(grid_for(tot)), dim3(NT), 0, st, W1, ldw, F, ldf, n, s, sp, mode,
+                     out, ldo);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_hadamard(const double* G1, const double* G2, const double* G3, int64_t s, int64_t ld, double* out,
+                hipStream_t st) {
+  hipLaunchKernelGGL(hadamard_kernel, dim3(grid_for(s * s)), dim3(NT), 0, st, G1, G2, G3, s, ld, out);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_balance_norm(float* A, int64_t lda, int64_t n, float* B, int64_t ldb, float* D, int64_t ldd, int64_t R,
+                    int64_t s, const int* skip, hipStream_t st) {
+  hipLaunchKernelGGL(balance_kernel, dim3((unsigned)s), dim3(NT), 0, st, A, lda, n, B, ldb, D, ldd, R, s, skip);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_sum3(const double* G1, const double* G2, const double* G3, int64_t s, int64_t ld, double* out,
+            hipStream_t st) {
+  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(NT), 0, st, G1, G2, G3, s, ld, out);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+// partial sums of squares; part must hold ceil(rows / rows_per_block) doubles
+int fh_sumsq_partial(const void* X, int dt, int64_t rows, int64_t cols, int64_t ld, int64_t rows_per_block,
+                     double* part, hipStream_t st) {
+  if (rows <= 0) return FH_OK;
+  int64_t nb = (rows + rows_per_block - 1) / rows_per_block;
+  FH_REQUIRE(nb < (1ll << 31), "sumsq: too many blocks");
+  hipLaunchKernelGGL(sumsq_partial_kernel, dim3((unsigned)nb), dim3(NT), 0, st, X, dt, rows, cols, ld,
+                     rows_per_block, part);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_segment_sum(const double* vals, const int64_t* offsets_dev, int n_seg, double* out, int accumulate,
+                   hipStream_t st) {
+  if (n_seg <= 0) return FH_OK;
+  hipLaunchKernelGGL(segment_sum_kernel, dim3(n_seg), dim3(NT), 0, st, vals, offsets_dev, n_seg, out, accumulate);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_dot2d(const void* X, int xdt, int64_t ldx, const void* Y, int ydt, int64_t ldy, int64_t rows,
+             int64_t cols, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(dot2d_kernel, dim3(1), dim3(NT), 0, st, X, xdt, ldx, Y, ydt, ldy, rows, cols, out);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_sweep_errors(const double* tnorm, const double* core, const double* xU, const double* xV, int K,
+                    double* hist, hipStream_t st) {
+  hipLaunchKernelGGL(sweep_errors_kernel, dim3(1), dim3(64), 0, st, tnorm, core, xU, xV, K, hist);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_cpals_check(const double* tnorm2, const double* inner, const double* xhat2, double* state, int* flag,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(cpals_check_kernel, dim3(1), dim3(64), 0, st, tnorm2, inner, xhat2, state, flag);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+}  // extern "C"

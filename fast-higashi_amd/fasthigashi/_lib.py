@@ -1,0 +1,163 @@
+"""ctypes binding of libfh_hip.so (the gfx950 kernels behind this package).
+
+The library is built in-tree by `__graft_entry__.build()` (make -C
+fast-higashi_amd/csrc).  There is no fallback: if the shared object is missing
+every compute call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int32, c_int64,
+                    c_uint64, c_void_p)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_native", "libfh_hip.so")
+
+F32, BF16, F64, FP8 = 0, 1, 2, 3
+GEMM_ATOMIC, GEMM_OUT_BF16 = 1, 2
+
+
+class GemmDesc(Structure):
+    _fields_ = [
+        ("A", c_void_p), ("B", c_void_p), ("C", c_void_p),
+        ("M", c_int64), ("N", c_int64), ("K", c_int64),
+        ("sAm", c_int64), ("sAk", c_int64), ("sBk", c_int64), ("sBn", c_int64),
+        ("sCm", c_int64), ("sCn", c_int64),
+        ("sAb", c_int64), ("sBb", c_int64), ("sCb", c_int64),
+        ("batch", c_int32), ("flags", c_int32),
+        ("alpha", c_double), ("beta", c_double), ("diag", c_double),
+        ("scale_a", c_void_p), ("scale_b", c_void_p), ("skip", c_void_p),
+        ("tile_begin", c_int64), ("tiles_m", c_int32), ("tiles_n", c_int32),
+    ]
+
+
+class NsGroup(Structure):
+    _fields_ = [
+        ("Y0", c_void_p), ("Z0", c_void_p), ("Y1", c_void_p), ("Z1", c_void_p), ("T", c_void_p),
+        ("scale", c_void_p), ("res", c_void_p),
+        ("s", c_int64), ("ld", c_int64), ("stride", c_int64), ("count", c_int64),
+        ("tile_begin", c_int64), ("tiles", c_int32), ("tm", c_int32),
+    ]
+
+
+class SpdSys(Structure):
+    _fields_ = [("G", c_void_p), ("out", c_void_p), ("s", c_int64), ("ldg", c_int64),
+                ("ldo", c_int64), ("eps", c_double)]
+
+
+class RowScale(Structure):
+    _fields_ = [("Y", c_void_p), ("A", c_void_p), ("nrows", c_int64), ("w", c_int64),
+                ("s", c_int64), ("ldy", c_int64), ("lda", c_int64)]
+
+
+class BandTile(Structure):
+    _fields_ = [("offset", c_int64), ("s_row", c_int64), ("s_col", c_int64), ("s_cell", c_int64),
+                ("row_start", c_int32), ("rows", c_int32), ("shift", c_int32), ("width", c_int32)]
+
+
+class LossTile(Structure):
+    _fields_ = [("X", c_void_p), ("P", c_void_p), ("Y", c_void_p), ("gY", c_void_p), ("gP", c_void_p),
+                ("w", c_int64), ("s", c_int64), ("n", c_int64), ("ldx", c_int64), ("ldp", c_int64),
+                ("ldy", c_int64), ("rows", c_int64), ("out_slot", c_int64)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "fh_last_error": (c_char_p, []),
+    "fh_clear_error": (None, []),
+    "fh_version": (c_int, []),
+    "fh_target": (c_char_p, []),
+    "fh_stream_sync": (c_int, [c_void_p]),
+    "fh_gemm_prepare": (c_int64, [c_void_p, c_int, c_int, c_int]),
+    "fh_gemm_shape_dims": (None, [c_int, POINTER(c_int), POINTER(c_int)]),
+    "fh_gemm_launch": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int64, c_void_p]),
+    "fh_gemm64_launch": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int64, c_void_p]),
+    "fh_ns_prepare": (c_int64, [c_void_p, c_int]),
+    "fh_ns_run": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int64, c_int, c_double, c_void_p,
+                          POINTER(c_int), c_void_p]),
+    "fh_spd_inverse": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p]),
+    "fh_fill": (c_int, [c_void_p, c_int, c_int64, c_uint64, c_void_p]),
+    "fh_copy2d": (c_int, [c_void_p, c_int, c_int64, c_int64, c_void_p, c_int, c_int64, c_int64, c_int64,
+                          c_double, c_void_p]),
+    "fh_scale_rows": (c_int, [c_void_p, c_int, c_int64, c_void_p]),
+    "fh_krp": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int,
+                       c_void_p]),
+    "fh_mttkrp_reduce": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int,
+                                 c_void_p, c_int64, c_void_p]),
+    "fh_hadamard": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    "fh_balance_norm": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_int64,
+                                c_int64, c_void_p, c_void_p]),
+    "fh_sum3": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    "fh_sumsq_partial": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "fh_segment_sum": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    "fh_dot2d": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p,
+                         c_void_p]),
+    "fh_sweep_errors": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "fh_cpals_check": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fh_csr_sort_workspace": (c_int64, [c_int64, c_int32, c_int64]),
+    "fh_csr_sort": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_int64, c_void_p]),
+    "fh_band_scatter": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int32, c_int32, c_void_p,
+                                c_int, c_void_p, c_void_p]),
+    "fh_transpose": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int, c_int64, c_void_p,
+                             c_void_p]),
+    "fh_rwr_prep": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p,
+                            c_int64, c_int64, c_int64, c_void_p]),
+    "fh_rwr_delta": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                             c_void_p]),
+    "fh_rwr_symnorm": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p]),
+    "fh_div_cols": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
+                            c_void_p]),
+    "fh_pool": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64,
+                        c_void_p]),
+    "fh_colsum_add": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64,
+                              c_void_p]),
+}
+
+_OPTIONAL = set()
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfh_hip.so once; raise if it is absent (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeError(
+            "libfh_hip.so not found at %s: build it with __graft_entry__.build() "
+            "(make -C fast-higashi_amd/csrc)" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        try:
+            f = getattr(L, name)
+        except AttributeError:
+            if name in _OPTIONAL:
+                continue
+            raise
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+def check(status: int, what: str = ""):
+    if status != 0:
+        msg = lib().fh_last_error().decode(errors="replace")
+        raise NativeError("%s failed (status %d): %s" % (what or "fh call", status, msg))
+
+
+def call(name: str, *args):
+    """Call a status-returning export and raise on failure."""
+    f = getattr(lib(), name)
+    check(f(*args), name)

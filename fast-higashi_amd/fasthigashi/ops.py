@@ -1,0 +1,291 @@
+"""Torch-facing wrappers of the gfx950 kernels.
+
+Tensors are device memory handed to the kernels as raw pointers; descriptor tables
+for grouped launches are packed once on the host and uploaded once, so a sweep
+re-launches them without host work (and can be captured in a hipGraph).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+DT_CODE = {torch.float32: L.F32, torch.bfloat16: L.BF16, torch.float64: L.F64, torch.uint8: L.FP8}
+
+# tile shapes of the MFMA kernel: index -> (BM, BN)
+SHAPES = {0: (128, 128), 1: (128, 64), 2: (64, 64), 3: (128, 160), 4: (64, 128), 5: (256, 128)}
+
+
+def stream_handle(stream: Optional[torch.cuda.Stream] = None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def ptr(t: torch.Tensor, offset_elems: int = 0) -> int:
+    return t.data_ptr() + offset_elems * t.element_size()
+
+
+def pick_shape(M: int, N: int) -> int:
+    """Default tile shape: cover N in one tile where possible (the streamed operand is
+    then read once), fall back to 64-row tiles for short M."""
+    if N <= 64:
+        return 1 if M > 64 else 2
+    if N <= 128:
+        return 0 if M > 64 else 4
+    if N <= 160:
+        return 3
+    return 0
+
+
+def make_desc(A: int, B: int, C: int, M: int, N: int, K: int, sAm: int, sAk: int, sBk: int, sBn: int,
+              sCm: int, sCn: int, batch: int = 1, sAb: int = 0, sBb: int = 0, sCb: int = 0,
+              alpha: float = 1.0, beta: float = 0.0, diag: float = 0.0, flags: int = 0, ksplit: int = 1,
+              skip: int = 0, scale_a: int = 0, scale_b: int = 0) -> L.GemmDesc:
+    d = L.GemmDesc()
+    d.A, d.B, d.C = A, B, C
+    d.M, d.N, d.K = M, N, K
+    d.sAm, d.sAk, d.sBk, d.sBn, d.sCm, d.sCn = sAm, sAk, sBk, sBn, sCm, sCn
+    d.sAb, d.sBb, d.sCb = sAb, sBb, sCb
+    d.batch = batch
+    d.flags = flags | ((ksplit if ksplit > 1 else 0) << 8)
+    d.alpha, d.beta, d.diag = alpha, beta, diag
+    d.scale_a, d.scale_b, d.skip = scale_a or None, scale_b or None, skip or None
+    return d
+
+
+def _upload(structs, device) -> torch.Tensor:
+    raw = bytes(b"".join(bytes(s) for s in structs))
+    host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
+    return host.to(device)
+
+
+class GemmGroup:
+    """One grouped launch of the fp32/bf16/fp8 MFMA GEMM over a fixed descriptor table."""
+
+    def __init__(self, descs: Sequence[L.GemmDesc], dtype: int, shape: Optional[int] = None,
+                 device="cuda"):
+        assert len(descs) > 0
+        self.dtype = dtype
+        d0 = descs[0]
+        self.ak = 1 if d0.sAk == 1 else 0
+        self.bk = 1 if d0.sBk == 1 else 0
+        vec = {L.F32: 4, L.BF16: 8, L.FP8: 16}[dtype]
+        for d in descs:
+            assert (d.sAk == 1) == bool(self.ak) and (d.sBk == 1) == bool(self.bk), "mixed layouts in one group"
+            assert d.sAk == 1 or d.sAm == 1, "A needs a unit stride"
+            assert d.sBk == 1 or d.sBn == 1, "B needs a unit stride"
+            # 16-byte vector loads along the unit-stride dimension: the other stride (the
+            # leading dimension) and the batch stride must be multiples of the vector width,
+            # and the base pointers 16-byte aligned, so every vector stays inside its row.
+            lda = d.sAm if self.ak else d.sAk
+            ldb = d.sBn if self.bk else d.sBk
+            for ld, bs, p, nm in ((lda, d.sAb, d.A, "A"), (ldb, d.sBb, d.B, "B")):
+                assert ld % vec == 0, "%s leading dim %d not a multiple of %d" % (nm, ld, vec)
+                assert d.batch == 1 or bs % vec == 0, "%s batch stride %d not a multiple of %d" % (nm, bs, vec)
+                assert p % 16 == 0, "%s pointer not 16-byte aligned" % nm
+            ext_a = d.K if self.ak else d.M
+            ext_b = d.K if self.bk else d.N
+            assert lda >= ext_a and ldb >= ext_b, "leading dims shorter than the unit-stride extents"
+        if shape is None:
+            shape = pick_shape(max(d.M for d in descs), max(d.N for d in descs))
+        self.shape = shape
+        bm, bn = SHAPES[shape]
+        arr = (L.GemmDesc * len(descs))(*descs)
+        self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), bm, bn)
+        self.n = len(descs)
+        self.descs = list(arr)
+        self._single = arr[0] if len(descs) == 1 else None
+        self.table = None if self._single is not None else _upload(arr, device)
+
+    def run(self, stream=None):
+        st = stream_handle(stream)
+        if self._single is not None:
+            L.call("fh_gemm_launch", self.dtype, self.shape, self.ak, self.bk, None, 1,
+                   ctypes.byref(self._single), self.total, st)
+        else:
+            L.call("fh_gemm_launch", self.dtype, self.shape, self.ak, self.bk, self.table.data_ptr(), self.n,
+                   None, self.total, st)
+
+
+class Gemm64Group:
+    """Grouped launch of the FP64 MFMA GEMM (operands fp32 or fp64, output fp32 or fp64)."""
+
+    def __init__(self, descs: Sequence[L.GemmDesc], a_dt: int, b_dt: int, c_dt: int, device="cuda"):
+        arr = (L.GemmDesc * len(descs))(*descs)
+        self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), 64, 64)
+        self.n = len(descs)
+        self.dts = (a_dt, b_dt, c_dt)
+        self._single = arr[0] if len(descs) == 1 else None
+        self.table = None if self._single is not None else _upload(arr, device)
+
+    def run(self, stream=None):
+        st = stream_handle(stream)
+        if self._single is not None:
+            L.call("fh_gemm64_launch", *self.dts, None, 1, ctypes.byref(self._single), self.total, st)
+        else:
+            L.call("fh_gemm64_launch", *self.dts, self.table.data_ptr(), self.n, None, self.total, st)
+
+
+def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alpha=1.0, beta=0.0,
+         shape=None, stream=None) -> torch.Tensor:
+    """C = alpha A @ B + beta C for 2-D (or 3-D batched) strided tensors on the device.
+
+    Inputs fp32 / bf16 (both the same) run on the MFMA kernel; fp64 inputs on the
+    FP64 kernel.  One of the two strides of each operand must be 1."""
+    batched = A.dim() == 3
+    if not batched:
+        A, B = A.unsqueeze(0), B.unsqueeze(0)
+        if C is not None:
+            C = C.unsqueeze(0)
+    nb, M, K = A.shape
+    N = B.shape[2]
+    if C is None:
+        C = torch.zeros((nb, M, N), dtype=torch.float32 if A.dtype != torch.float64 else torch.float64,
+                        device=A.device)
+    d = make_desc(A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, A.stride(1), A.stride(2),
+                  B.stride(1), B.stride(2), C.stride(1), C.stride(2), nb, A.stride(0), B.stride(0),
+                  C.stride(0), alpha, beta)
+    if A.dtype == torch.float64 or B.dtype == torch.float64 or C.dtype == torch.float64:
+        Gemm64Group([d], DT_CODE[A.dtype], DT_CODE[B.dtype], DT_CODE[C.dtype]).run(stream)
+    else:
+        assert A.dtype == B.dtype
+        GemmGroup([d], DT_CODE[A.dtype], shape).run(stream)
+    return C if batched else C[0]
+
+
+def fill_(t: torch.Tensor, value: float, stream=None) -> torch.Tensor:
+    """Fill a contiguous tensor by a kernel (no hipMemsetAsync, so it is graph-safe)."""
+    if t.numel() == 0:
+        return t
+    if t.dtype == torch.float32:
+        pat = int(np.array([value], dtype=np.float32).view(np.uint32)[0]); es = 4
+    elif t.dtype == torch.float64:
+        pat = int(np.array([value], dtype=np.float64).view(np.uint64)[0]); es = 8
+    elif t.dtype == torch.bfloat16:
+        pat = int(torch.tensor([value], dtype=torch.bfloat16).view(torch.int16).item()) & 0xFFFF; es = 2
+    elif t.dtype in (torch.int32,):
+        pat = int(value) & 0xFFFFFFFF; es = 4
+    elif t.dtype in (torch.int64,):
+        pat = int(value) & 0xFFFFFFFFFFFFFFFF; es = 8
+    elif t.dtype in (torch.uint8, torch.int8):
+        pat = int(value) & 0xFF; es = 1
+    else:
+        raise TypeError(t.dtype)
+    L.call("fh_fill", t.data_ptr(), es, t.numel(), pat, stream_handle(stream))
+    return t
+
+
+def copy2d(src: torch.Tensor, dst: torch.Tensor, alpha: float = 1.0, stream=None):
+    """dst[r, c] = alpha * src[r, c] with dtype conversion (fp32/fp64/bf16), 2-D strided src."""
+    assert src.dim() == 2 and dst.dim() == 2 and dst.stride(1) == 1
+    L.call("fh_copy2d", src.data_ptr(), DT_CODE[src.dtype], src.stride(0), src.stride(1), dst.data_ptr(),
+           DT_CODE[dst.dtype], dst.stride(0), src.shape[0], src.shape[1], alpha, stream_handle(stream))
+
+
+def rup(x: int, m: int) -> int:
+    return (x + m - 1) // m * m
+
+
+class PolarBatch:
+    """Batched FP64 polar factors of groups of same-size matrices.
+
+    For each group g: `count` matrices Y (m_g x s_g); the caller writes G = Y^T Y into
+    `self.G(g)` (FP64, s x s with leading dim ld), runs `solve()`, then reads
+    G^{-1/2} from `self.Zinv(g)` and the nuclear norms from `self.nuc`."""
+
+    def __init__(self, sizes: Sequence[int], counts: Sequence[int], n_iter: int = 48, tol: float = 1e-24,
+                 device="cuda"):
+        self.sizes = list(sizes)
+        self.counts = list(counts)
+        self.n_iter = n_iter
+        self.tol = tol
+        self.lds = [rup(s, 4) for s in sizes]
+        self.strides = [ld * s for ld, s in zip(self.lds, sizes)]
+        tot = sum(st * c for st, c in zip(self.strides, counts))
+        nmat = sum(counts)
+        self.nmat = nmat
+        self.buf = torch.zeros(5 * tot, dtype=torch.float64, device=device)
+        self.scale = torch.zeros(nmat, dtype=torch.float64, device=device)
+        self.res = torch.zeros(n_iter * nmat, dtype=torch.float64, device=device)
+        self.nuc = torch.zeros(nmat, dtype=torch.float64, device=device)
+        self.offs = []
+        groups = []
+        off = 0
+        moff = 0
+        mat_begin = []
+        for s, c, ld, st in zip(sizes, counts, self.lds, self.strides):
+            g = L.NsGroup()
+            n = st * c
+            base = self.buf.data_ptr()
+            g.Y0 = base + 8 * (off)
+            g.Z0 = base + 8 * (tot + off)
+            g.Y1 = base + 8 * (2 * tot + off)
+            g.Z1 = base + 8 * (3 * tot + off)
+            g.T = base + 8 * (4 * tot + off)
+            g.scale = self.scale.data_ptr() + 8 * moff
+            g.res = self.res.data_ptr() + 8 * (n_iter * moff)
+            g.s, g.ld, g.stride, g.count = s, ld, st, c
+            self.offs.append((off, moff))
+            groups.append(g)
+            mat_begin.append(moff)
+            off += n
+            moff += c
+        self.tot = tot
+        arr = (L.NsGroup * len(groups))(*groups)
+        self.total_tiles = L.lib().fh_ns_prepare(ctypes.cast(arr, ctypes.c_void_p), len(groups))
+        self.groups = list(arr)
+        self.table = _upload(arr, device)
+        self.mat_begin = torch.tensor(mat_begin, dtype=torch.int64, device=device)
+        self.result = 0
+
+    def G(self, g: int) -> torch.Tensor:
+        off, _ = self.offs[g]
+        return self.buf[off: off + self.strides[g] * self.counts[g]].view(self.counts[g], self.sizes[g], self.lds[g])
+
+    def Zinv(self, g: int) -> torch.Tensor:
+        off, _ = self.offs[g]
+        base = (3 if self.result else 1) * self.tot
+        return self.buf[base + off: base + off + self.strides[g] * self.counts[g]].view(
+            self.counts[g], self.sizes[g], self.lds[g])
+
+    def nuc_of(self, g: int) -> torch.Tensor:
+        _, m = self.offs[g]
+        return self.nuc[m: m + self.counts[g]]
+
+    def solve(self, stream=None):
+        r = ctypes.c_int(0)
+        L.call("fh_ns_run", self.table.data_ptr(), len(self.groups), self.mat_begin.data_ptr(),
+               self.total_tiles, self.nmat, self.n_iter, self.tol, self.nuc.data_ptr(), ctypes.byref(r),
+               stream_handle(stream))
+        self.result = r.value
+
+
+def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24):
+    """Polar factor of a batch of fp32 matrices (b, m, s), m >= s, via FP64 Gram + Newton-Schulz.
+
+    Returns (P fp32 (b, m, s), nuclear norms fp64 (b,))."""
+    squeeze = Y.dim() == 2
+    if squeeze:
+        Y = Y.unsqueeze(0)
+    b, m, s = Y.shape
+    pb = PolarBatch([s], [b], n_iter, tol, device=Y.device)
+    G = pb.G(0)
+    Yc = Y.contiguous()
+    d = make_desc(Yc.data_ptr(), Yc.data_ptr(), G.data_ptr(), s, s, m, 1, s, s, 1, G.stride(1), 1, b,
+                  m * s, m * s, G.stride(0))
+    Gemm64Group([d], DT_CODE[Yc.dtype], DT_CODE[Yc.dtype], L.F64).run()
+    pb.solve()
+    Z = pb.Zinv(0)
+    P = torch.empty((b, m, s), dtype=torch.float32, device=Y.device)
+    d2 = make_desc(Yc.data_ptr(), Z.data_ptr(), P.data_ptr(), m, s, s, s, 1, 1, Z.stride(1), s, 1, b, m * s,
+                   Z.stride(0), m * s)
+    Gemm64Group([d2], DT_CODE[Yc.dtype], L.F64, L.F32).run()
+    nuc = pb.nuc_of(0).clone()
+    if squeeze:
+        return P[0], nuc[0]
+    return P, nuc

@@ -1,0 +1,531 @@
+"""numpy restatement of Fast-Higashi's decomposition path (test oracle only).
+
+Every function cites the reference lines it restates.  `dt` selects the float
+type: np.float32 follows the reference op for op, np.float64 is the FP64 oracle.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+
+# ---------------------------------------------------------------------------
+# A1  COO build (FastHigashi_Wrapper.py:221-366, preprocessing.py:137-142,474-489)
+# ---------------------------------------------------------------------------
+
+def pack_chrom(mats, off_diag: int):
+    """Per-cell symmetric float32 CSR matrices of one chromosome -> (indices, values, shape).
+
+    Restates pack_training_data_one_process with merge_fac = 1, fac_size = 1,
+    no blacklist, no batch_id (FastHigashi_Wrapper.py:253-366).
+    """
+    matrix_list = [m.tocoo() for m in mats]
+    for m in matrix_list:                                     # :265-269
+        keep = np.abs(m.col - m.row) <= off_diag
+        m.row = m.row[keep]
+        m.col = m.col[keep]
+        m.data = m.data[keep]
+    bulk = np.zeros(matrix_list[0].shape)                     # :278, :421-426
+    for a in matrix_list:
+        bulk[a.row, a.col] += a.data
+    bulk = bulk / len(matrix_list)                            # :280
+    rowsum = bulk.sum(1)                                      # preprocessing.py:477-484
+    valid = rowsum > min(0., 0.01 * bulk.shape[1])
+    mapping = np.cumsum(valid) - 1
+    mapping[~valid] = -1
+    n_valid = int(valid.sum())
+    for m in matrix_list:                                     # :293-298 -> preprocessing.py:137-142
+        scale = m.shape[0]
+        n = m.sum()
+        m.data *= scale / (n + 1e-15)
+    nnz = sum(m.nnz for m in matrix_list)                     # :301-347
+    indices = np.empty([3, nnz], dtype=np.int32)
+    values = np.empty([nnz], dtype=np.float32)
+    k = 0
+    for i, m in enumerate(matrix_list):
+        if m.nnz == 0:
+            continue
+        row, col, data = mapping[m.row], mapping[m.col], m.data
+        col_new = col - row
+        ok = (row != -1) & (col != -1)
+        row, col, data = row[ok], col_new[ok], data[ok]
+        col = col + row
+        cnt = len(data)
+        indices[0, k:k + cnt] = row
+        indices[1, k:k + cnt] = col
+        indices[2, k:k + cnt] = i
+        values[k:k + cnt] = data
+        k += cnt
+    indices = np.ascontiguousarray(indices[:, :k])
+    values = np.ascontiguousarray(values[:k])
+    shape = (n_valid, n_valid, len(matrix_list))
+    assert indices.min() >= 0 and (indices.max(1) < shape).all()   # :353-354
+    values = np.log1p(values)                                 # :358-364
+    mean_, std_ = np.mean(values), np.std(values)
+    values = np.clip(values, a_min=None, a_max=mean_ + 15 * std_)
+    return indices, values, shape, mapping
+
+
+# ---------------------------------------------------------------------------
+# A2/A3  CSR by row, row-range view (sparse_for_schic.py:133-213)
+# ---------------------------------------------------------------------------
+
+def sort_indices(indices, values, shape):
+    """Stable sort of COO entries by row plus indptr (sparse_for_schic.py:133-182).
+
+    coo_tocsr places entries row by row in input order, so this is a stable
+    counting sort on the row index.
+    """
+    row = indices[0]
+    n_rows = int(shape[0])
+    if len(row) == 0 or (np.diff(row) >= 0).all():           # :137-151 fast path
+        order = np.arange(len(row))
+    else:
+        order = np.argsort(row, kind="stable")
+    indices = np.ascontiguousarray(indices[:, order])
+    values = np.ascontiguousarray(values[order])
+    indptr = np.searchsorted(indices[0], np.arange(n_rows + 1), side="left").astype(np.int64)
+    return indices, values, indptr
+
+
+def slice_rows(indices, values, indptr, shape, start, stop):
+    """Entries of rows [start, stop) (sparse_for_schic.py:200-213)."""
+    stop = min(stop, int(shape[0]))
+    lo, hi = int(indptr[start]), int(indptr[stop])
+    return indices[:, lo:hi], values[lo:hi], (stop - start,) + tuple(shape[1:])
+
+
+# ---------------------------------------------------------------------------
+# A4/A5/A6  tiling, padded tile COO, dense tile (sparse_for_schic.py:279-510)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Tile:
+    start: int          # first row bin (bin_slice_list[t].start)
+    rows: int           # bin_slice_list[t] length
+    width: int          # compact tile width w
+    col_start: int      # col_bin_slice_list[t].start
+    local_start: int    # local_bin_slice_list[t].start
+    shift: int          # subtracted from absolute columns
+
+
+def tile_plan(n_bins: int, bs_bin: int, flank: int) -> List[Tile]:
+    """Compact band tiles of one chromosome (sparse_for_schic.py:401-510, compact=True)."""
+    tiles = []
+    for i in range(0, n_bins, bs_bin):
+        rows = min(i + bs_bin, n_bins) - i
+        right = flank if n_bins - i - rows - flank > 0 else n_bins - i - rows
+        if i > flank:
+            left, local_start, col_start, shift = flank, flank, i - flank, i - flank
+        else:
+            left, local_start, col_start, shift = i, i, 0, 0
+        tiles.append(Tile(i, rows, rows + left + right, col_start, local_start, shift))
+    return tiles
+
+
+def cell_slices(num_cell: int, total: int, bs_cell: int):
+    """cell_slice_list of Chrom_Dataset (sparse_for_schic.py:422-439)."""
+    pts = list(range(0, num_cell, bs_cell)) + list(range(num_cell, total, bs_cell))
+    return [slice(p, min(p + bs_cell, num_cell if p < num_cell else total)) for p in pts]
+
+
+def tile_coo(indices, values, indptr, shape, tile: Tile, cells: slice):
+    """Fake_Sparse contents of one (tile, cell chunk): +1 padded rows/cols
+    (sparse_for_schic.py:409,419,452-499, :327-339)."""
+    idx, val, _ = slice_rows(indices, values, indptr, shape, tile.start, tile.start + tile.rows)
+    m = (idx[2] >= cells.start) & (idx[2] < cells.stop)
+    r = idx[0][m] - tile.start + 1
+    c = idx[1][m] - tile.shift + 1
+    k = idx[2][m] - cells.start
+    return (r.astype(np.int32), c.astype(np.int32), k.astype(np.int32)), val[m], \
+        (tile.rows + 2, tile.width + 2, cells.stop - cells.start)
+
+
+def densify(coo, vals, pshape, transpose: bool, dt=np.float32):
+    """densify_jit without do_conv (sparse_for_schic.py:279-320)."""
+    r, c, k = coo
+    if transpose:
+        d = np.zeros((pshape[2], pshape[0], pshape[1]), dtype=dt)
+        d[k, r, c] = vals
+        d = d[:, 1:-1, 1:-1]
+    else:
+        d = np.zeros(pshape, dtype=dt)
+        d[r, c, k] = vals
+        d = d[1:-1, 1:-1, :]
+    return np.maximum(d, dt(1e-8))
+
+
+# ---------------------------------------------------------------------------
+# A7  partial RWR (partial_rwr.py:45-175)
+# ---------------------------------------------------------------------------
+
+def avg_pool3(x):
+    """F.avg_pool2d(x, 3, 1, padding=1, ceil_mode=True) with count_include_pad=True."""
+    p = np.pad(x, ((0, 0), (1, 1), (1, 1)))
+    out = np.zeros_like(x)
+    for di in range(3):
+        for dj in range(3):
+            out += p[:, di:di + x.shape[1], dj:dj + x.shape[2]]
+    return out / x.dtype.type(9)
+
+
+def partial_rwr(x, slice_start, slice_end, do_conv, do_rwr, do_col,
+                bin_cov=None, force_rwr_epochs=-1, final_transpose=True, rp=0.5):
+    """x: (cells, rows, w).  Returns (out, n_iter) like partial_rwr.py:45-175."""
+    dt = x.dtype.type
+    n_iter = 0
+    if do_conv or do_rwr:
+        if do_conv and x.shape[1] > 1:                              # :77-81
+            x = np.maximum(avg_pool3(x), dt(1e-8))
+        if do_rwr:
+            A = x
+            s2 = A @ A.transpose(0, 2, 1)                           # :85
+            idx = np.arange(s2.shape[1])
+            s2[:, idx, idx] = 0                                     # :86-87
+            s1 = A[:, :, slice_start:slice_end].copy()              # :88
+            s2 = (s2 / (s2.sum(1, keepdims=True) + dt(1e-15))) * dt(0.25)   # :90
+            s1 = (s1 / (s1.sum(1, keepdims=True) + dt(1e-15))) * dt(0.75)   # :91
+            local = s1 + s2                                         # :93
+            zero_col = (local.sum(1) == 0)                          # :96-97
+            local[:, idx, idx] += zero_col.astype(dt)
+            if force_rwr_epochs < 0:                                # :99-104
+                auto, epochs = True, 60
+            else:
+                auto, epochs = False, force_rwr_epochs
+            P = local / (local.sum(1, keepdims=True) + dt(1e-15))   # :111
+            Q = np.broadcast_to(np.eye(P.shape[1], dtype=x.dtype), P.shape).copy()
+            count = 0
+            for _ in range(epochs):                                 # :115-126
+                Qn = dt(rp) * (Q @ P)
+                Qn[:, idx, idx] += dt(1 - rp)
+                if auto:
+                    delta = np.sqrt(np.square(Q - Qn).sum(1).sum(1))
+                    Q = Qn
+                    if np.max(delta) < 0.01:
+                        break
+                else:
+                    Q = Qn
+                count += 1
+            local = Q
+            n_iter = count
+            if do_col:                                              # :131-135
+                local = (local + local.transpose(0, 2, 1)) * dt(0.5)
+                local = np.maximum(local, dt(0))
+                local = local / (local.sum(2, keepdims=True) + dt(1e-15))
+                A = A / bin_cov[:, None, :].astype(x.dtype)
+            x = local @ A                                           # :138
+        if final_transpose:                                         # :171-172
+            x = x.transpose(1, 2, 0)
+    return x, n_iter
+
+
+# ---------------------------------------------------------------------------
+# A8  polar factor (project2orthogonal.py:6-55)
+# ---------------------------------------------------------------------------
+
+def project2orthogonal(matrix, rank=None):
+    """Polar factor U[..., :rank] Vh[..., :rank, :] and singular values."""
+    if rank is None:
+        rank = min(matrix.shape[-2:])
+    U, S, Vh = np.linalg.svd(matrix, full_matrices=False)
+    return U[..., :rank] @ Vh[..., :rank, :], S[..., :rank]
+
+
+def project2orthogonal_eigh(matrix, rank=None, kk=1e-2):
+    """eigh fallback branch (project2orthogonal.py:33-55)."""
+    if rank is None:
+        rank = min(matrix.shape[-2:])
+    dim_1, dim_2 = matrix.shape[-2], matrix.shape[-1]
+    mode = dim_2 > dim_1
+    mT = np.swapaxes(matrix, -1, -2)
+    X = matrix @ mT if mode else mT @ matrix
+    X = X + kk * np.eye(X.shape[-1], dtype=X.dtype)
+    _, vecs = np.linalg.eigh(X)
+    vecs = vecs[..., -rank:][..., ::-1]
+    if mode:
+        U = vecs
+        Vh = np.swapaxes(U, -1, -2) @ matrix
+        V, R = np.linalg.qr(np.swapaxes(Vh, -1, -2))
+        V = V * np.sign(np.diagonal(R, axis1=-2, axis2=-1))[..., None, :]
+    else:
+        V = vecs
+        U, R = np.linalg.qr(matrix @ V)
+        U = U * np.sign(np.diagonal(R, axis1=-2, axis2=-1))[..., None, :]
+    return U @ np.swapaxes(V, -1, -2), None
+
+
+# ---------------------------------------------------------------------------
+# A15  inner CP-ALS (parafac_integrative.py:12-112)
+# ---------------------------------------------------------------------------
+
+def balance_norm(factors):
+    """parafac_integrative.py:19-26."""
+    dt = factors[0].dtype.type
+    norm = 1
+    for f in factors:
+        norm = norm * np.linalg.norm(f, axis=0)
+    even = norm + dt(1e-15)
+    for i in range(len(factors)):
+        factors[i] = factors[i] / (np.linalg.norm(factors[i], axis=0) + dt(1e-15))
+    factors[-1] = factors[-1] * even
+
+
+def update_factor(G, M):
+    """f = solve(G + 1e-10 I, M^T)^T (parafac_integrative.py:12-15)."""
+    G = G.copy()
+    idx = np.arange(G.shape[0])
+    G[idx, idx] += G.dtype.type(1e-10)
+    return np.linalg.solve(G, M.T).T
+
+
+def parafac(X, rank, n_iter_max, init):
+    """CP-ALS of a 3-way tensor (n, s, R) (parafac_integrative.py:28-112).
+
+    Returns (factors, loss_norm, loss_x, history)."""
+    dt = X.dtype.type
+    factors = [f.astype(X.dtype).copy() for f in init]
+    balance_norm(factors)                                           # :65
+    history = []
+    for _ in range(n_iter_max):
+        for i in range(3):
+            G = np.ones((rank, rank), dtype=X.dtype)                 # :43-48
+            for j, f in enumerate(factors):
+                if j != i:
+                    G = G * (f.T @ f)
+            M = mttkrp(X, factors, i)                               # :55-62
+            factors[i] = update_factor(G, M)
+        Xhat = np.einsum("ir,jr,kr->ijk", *factors)                 # :82-86
+        loss_recon = float(np.square(X - Xhat).sum())
+        loss_norm = float(np.square(Xhat).sum())
+        loss_x = float((Xhat * X).sum())
+        history.append(dict(loss=loss_recon, norm=loss_norm, x=loss_x))
+        wdiff = np.nan if len(history) < 2 else \
+            (history[-2]["loss"] - history[-1]["loss"]) / history[-2]["loss"]
+        if wdiff < 1e-5:                                            # :107
+            break
+        balance_norm(factors)                                       # :108
+    return factors, history[-1]["norm"], history[-1]["x"], history
+
+
+def mttkrp(X, factors, mode):
+    A, B, D = factors
+    if mode == 0:
+        return np.einsum("ijk,jr,kr->ir", X, B, D, optimize=True)
+    if mode == 1:
+        return np.einsum("ijk,ir,kr->jr", X, A, D, optimize=True)
+    return np.einsum("ijk,ir,jr->kr", X, A, B, optimize=True)
+
+
+# ---------------------------------------------------------------------------
+# A4/A16/A19  dataset and sizes
+# ---------------------------------------------------------------------------
+
+@dataclass
+class ChromData:
+    """Post-RWR dense tiles of one chromosome (what fetch + partial_rwr yield).
+
+    X[t] has shape (rows_t, w_t, N) and holds the good-QC cells."""
+    chrom: str
+    num_bin: int
+    resolution: int
+    tiles: List[Tile]
+    X: List[np.ndarray]
+    bs_bin: int
+    cell_chunks: List[slice]
+
+
+def size_of(num_bin, size_ratio, resolution, rank):
+    """Per-chromosome CP rank (parafac2_intergrative.py:558-560)."""
+    return min(int(num_bin * size_ratio * resolution / 1000000), rank)
+
+
+def recommend_bs_bin(resolution):
+    return min(max(int(15000000 / resolution), 128), 256)      # FastHigashi_Wrapper.py:501
+
+
+def gpu_tiling(num_bin, resolution, off_diag, num_cell, avail_mem):
+    """bs_bin / bs_cell of the reference's device branch (FastHigashi_Wrapper.py:497-517)."""
+    rec = recommend_bs_bin(resolution)
+    n_batch = max(math.ceil(num_bin / rec), 1)
+    bs_bin = math.ceil(num_bin / n_batch)
+    max_tensor_size = avail_mem / (4 * 12)
+    bs_cell = int(max_tensor_size / (bs_bin * (bs_bin + 2 * off_diag)))
+    n_batch = int(math.ceil(num_cell / bs_cell))
+    bs_cell = int(math.ceil(num_cell / n_batch))
+    bs_cell = min(bs_cell, num_cell)
+    return bs_bin, bs_cell
+
+
+# ---------------------------------------------------------------------------
+# A10-A14  one sweep (parafac2_intergrative.py:303-540)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Factors:
+    A: List[np.ndarray]          # (n_c, s_c)
+    B: List[np.ndarray]          # (s_c, s_c)
+    D: List[np.ndarray]          # (R, s_c)
+    M: np.ndarray                # (N, R)
+
+    def copy(self, dt=None):
+        c = (lambda a: a.astype(dt) if dt is not None else a.copy())
+        return Factors([c(a) for a in self.A], [c(b) for b in self.B],
+                       [c(d) for d in self.D], c(self.M))
+
+
+def core_norm(A, B, D, bs_bin):
+    """Sum over tiles of ||[[A_t, B, D]]||^2 (parafac2_intergrative.py:621-632)."""
+    tot = 0.0
+    for i in range(0, A.shape[0], bs_bin):
+        c = np.einsum("ir,jr,kr->kij", A[i:i + bs_bin], B, D)
+        tot += float(np.square(np.linalg.norm(c)))
+    return tot
+
+
+def sweep(data: List[ChromData], F: Factors, first_iter: bool, dt=np.float32):
+    """update_meta_embedding_interactions (parafac2_intergrative.py:303-540).
+
+    Returns (projections, projected, x_U, x_V, tensor_norm or None, M_new)."""
+    R = F.M.shape[1]
+    N = F.M.shape[0]
+    svd_term = np.zeros((R, N), dtype=dt)
+    x_U = np.zeros(len(data))
+    tnorm = np.zeros(len(data)) if first_iter else None
+    projections = []
+    for c, cd in enumerate(data):
+        A, B, D = F.A[c].astype(dt), F.B[c].astype(dt), F.D[c].astype(dt)
+        C = F.M.astype(dt) @ D                                        # :336
+        proj_c = []
+        for t, tile in enumerate(cd.tiles):
+            X = cd.X[t].astype(dt)
+            rows = slice(tile.start, tile.start + tile.rows)
+            temp = None
+            for cells in cd.cell_chunks:                              # :345-386
+                Xk = X[:, :, cells]
+                if first_iter:
+                    tnorm[c] += float(np.square(np.linalg.norm(Xk)))  # :368-369
+                lhs = (A[rows][:, None, :] * C[cells][None, :, :]) @ B.T   # 'ir,jr,kr->ikj'
+                part = Xk @ lhs
+                temp = part if temp is None else temp + part
+            U, S = project2orthogonal(temp, temp.shape[-1])           # :396
+            proj_c.append(U)
+            x_U[c] += float(S.sum())                                  # :417
+            lhs2 = np.einsum("ir,jr,kr->kij", A[rows], B, D).reshape(R, -1)   # :422-423
+            for cells in cd.cell_chunks:                              # :429-471
+                Xk = X[:, :, cells]
+                projected = np.swapaxes(U, -1, -2) @ Xk
+                svd_term[:, cells] += lhs2 @ projected.reshape(-1, projected.shape[-1])
+        projections.append(proj_c)
+    M_new, S = project2orthogonal(svd_term.T, R)                      # :483
+    x_V = float((M_new * svd_term.T).sum())                           # :486
+    projected_all = []
+    for c, cd in enumerate(data):                                     # :488-529
+        s = F.A[c].shape[1]
+        T = np.zeros((cd.num_bin, s, R), dtype=dt)
+        for t, tile in enumerate(cd.tiles):
+            X = cd.X[t].astype(dt)
+            U = projections[c][t]
+            g = 0
+            for cells in cd.cell_chunks:
+                Xk = X[:, :, cells]
+                g = g + np.swapaxes(U, -1, -2) @ (Xk @ M_new[cells])  # 'ijk,km,ijl->ilm'
+            T[tile.start:tile.start + tile.rows] = g
+        projected_all.append(T)
+    return projections, projected_all, x_U, x_V, tnorm, M_new
+
+
+@dataclass
+class FitResult:
+    factors: Factors
+    projections: list
+    rec_errors: List[float] = field(default_factory=list)
+    rec_errors_by_chrom: List[np.ndarray] = field(default_factory=list)
+    x_U: List[np.ndarray] = field(default_factory=list)
+    x_V: List[float] = field(default_factory=list)
+    core_norms: List[np.ndarray] = field(default_factory=list)
+    tensor_norm: Optional[np.ndarray] = None
+    iterations: int = 0
+
+
+def fit(data: List[ChromData], F0: Factors, n_iter_max: int, n_iter_parafac: int,
+        tol: float = 0.0, dt=np.float32) -> FitResult:
+    """Sweep loop of Fast_Higashi_core.fit (parafac2_intergrative.py:596-740).
+
+    One chromosome per dataset (single resolution), so chrom2id maps to itself."""
+    F = F0.copy(dt)
+    core = np.array([core_norm(F.A[c], F.B[c], F.D[c], cd.bs_bin) for c, cd in enumerate(data)])
+    res = FitResult(F, None)
+    tnorm = None
+    rec_total, rec_block = [], []
+    for it in range(n_iter_max):
+        if it % 10 == 0 and it > 0 and n_iter_parafac < 10:          # :636-637
+            n_iter_parafac += 1
+        proj, projected, x_U, x_V, tn, M_new = sweep(data, F, tnorm is None, dt)
+        if tnorm is None:
+            tnorm = tn
+            norm_tensor = np.sqrt(tnorm)
+            norm_all = float(np.linalg.norm(norm_tensor))
+        F.M = M_new.astype(dt)
+        err_U = tnorm + core - 2 * x_U                                # :667-668
+        err_V = tnorm.sum() + core.sum() - 2 * x_V
+        for c, cd in enumerate(data):                                 # :674-695
+            s = F.A[c].shape[1]
+            fac, _, _, _ = parafac(projected[c], s, n_iter_parafac, [F.A[c], F.B[c], F.D[c]])
+            F.A[c], F.B[c], F.D[c] = fac[0].astype(dt), fac[1].astype(dt), fac[2].astype(dt)
+        core = np.array([core_norm(F.A[c], F.B[c], F.D[c], cd.bs_bin) for c, cd in enumerate(data)])
+        rec = math.sqrt(err_V) / norm_all                             # :710
+        rb = np.sqrt(err_U) / norm_tensor
+        rec_total.append(rec)
+        rec_block.append(rb)
+        res.x_U.append(x_U)
+        res.x_V.append(x_V)
+        res.core_norms.append(core.copy())
+        res.iterations = it + 1
+        if it >= 1:                                                   # :715-730
+            diffs = (rec_block[-2] ** 2 - rec_block[-1] ** 2) / rec_block[-2] ** 2
+            tdiff = (rec_total[-2] ** 2 - rec_total[-1] ** 2) / rec_total[-2] ** 2
+            if it >= 3 and tol > 0 and (tdiff < tol or diffs.max() < tol * 2):
+                break
+    res.factors = F
+    res.projections = proj
+    res.rec_errors = rec_total
+    res.rec_errors_by_chrom = rec_block
+    res.tensor_norm = tnorm
+    return res
+
+
+def transform(data: List[ChromData], F: Factors, projections, dt=np.float32):
+    """Final meta-embedding over all cells (parafac2_intergrative.py:742-834)."""
+    R = F.M.shape[1]
+    N = F.M.shape[0]
+    svd_term = np.zeros((R, N), dtype=dt)
+    for c, cd in enumerate(data):
+        A, B, D = F.A[c].astype(dt), F.B[c].astype(dt), F.D[c].astype(dt)
+        for t, tile in enumerate(cd.tiles):
+            rows = slice(tile.start, tile.start + tile.rows)
+            lhs = np.einsum("ir,jr,kr->kij", A[rows], B, D).reshape(R, -1)
+            U = projections[c][t].astype(dt)
+            X = cd.X[t].astype(dt)
+            for cells in cd.cell_chunks:
+                projected = np.swapaxes(U, -1, -2) @ X[:, :, cells]
+                svd_term[:, cells] += lhs @ projected.reshape(-1, projected.shape[-1])
+    M, _ = project2orthogonal(svd_term.T, R)
+    return M
+
+
+def parafac2_loss(data: List[ChromData], F: Factors, projections, dt=np.float64):
+    """sum_c sum_i ||X_i - P_i B diag(A_i) C^T||^2 evaluated directly."""
+    tot = 0.0
+    for c, cd in enumerate(data):
+        A, B, D = F.A[c].astype(dt), F.B[c].astype(dt), F.D[c].astype(dt)
+        C = F.M.astype(dt) @ D
+        for t, tile in enumerate(cd.tiles):
+            rows = slice(tile.start, tile.start + tile.rows)
+            P = projections[c][t].astype(dt)
+            core = (B[None, :, :] * A[rows][:, None, :]) @ C.T        # (rows, s, N)
+            rec = P @ core
+            tot += float(np.square(cd.X[t].astype(dt) - rec).sum())
+    return tot

@@ -1,0 +1,211 @@
+"""Kernel-level parity of the HIP library against plain fp32/fp64 references."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _operand(rows, cols, unit, dtype, gen, device):
+    """rows x cols device matrix with unit stride along 'cols' (unit=1) or 'rows' (unit=0);
+    the unit-stride extent is padded to a multiple of 8 in storage."""
+    if unit == 1:
+        ld = -(-cols // 8) * 8
+        return torch.randn(rows, ld, generator=gen).to(dtype).to(device)[:, :cols]
+    ld = -(-rows // 8) * 8
+    return torch.randn(cols, ld, generator=gen).to(dtype).to(device)[:, :rows].t()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("ak,bk", [(1, 1), (1, 0), (0, 1), (0, 0)])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
+def test_gemm_layouts(cuda, dtype, ak, bk, shape):
+    from fasthigashi import ops
+    gen = torch.Generator().manual_seed(7 + shape)
+    M, N, K = 203, 150, 333
+    A = _operand(M, K, 1 if ak else 0, dtype, gen, cuda)            # M x K
+    B = _operand(K, N, 0 if bk else 1, dtype, gen, cuda)            # K x N (k-major: unit stride along K)
+    assert (A.stride(1) == 1) == bool(ak) and (B.stride(0) == 1) == bool(bk)
+    ref = A.double().cpu() @ B.double().cpu()
+    C = ops.gemm(A, B, shape=shape)
+    torch.cuda.synchronize()
+    err = (C.double().cpu() - ref).abs().max().item() / ref.abs().max().item()
+    assert err < 1e-5, (err, dtype, ak, bk, shape)
+
+
+def test_gemm_strides_preserved(cuda):
+    # .to(cuda) of a non-contiguous view keeps its strides? make sure the kernel sees a real transposed layout
+    from fasthigashi import ops
+    gen = torch.Generator().manual_seed(3)
+    At = torch.randn(64, 200, generator=gen).to(cuda)      # K x M storage
+    A = At.t()[:199]                                         # M x K, unit stride along M
+    B = torch.randn(64, 96, generator=gen).to(cuda)          # K x N, unit stride along N
+    assert A.stride(0) == 1
+    C = ops.gemm(A, B)
+    ref = A.double().cpu() @ B.double().cpu()
+    assert (C.double().cpu() - ref).abs().max().item() < 1e-4 * ref.abs().max().item()
+
+
+def test_gemm_batched_beta_diag(cuda):
+    from fasthigashi import ops, _lib as L
+    gen = torch.Generator().manual_seed(5)
+    b, M, N, K = 5, 70, 40, 88
+    A = torch.randn(b, M, K, generator=gen).to(cuda)
+    B = torch.randn(b, K, N, generator=gen).to(cuda)
+    C0 = torch.randn(b, M, N, generator=gen).to(cuda)
+    C = C0.clone()
+    d = ops.make_desc(A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, K, 1, N, 1, N, 1, b, M * K, K * N,
+                      M * N, alpha=0.5, beta=2.0, diag=1.5)
+    ops.GemmGroup([d], L.F32).run()
+    ref = 0.5 * (A.double() @ B.double()) + 2.0 * C0.double()
+    idx = torch.arange(min(M, N))
+    ref[:, idx, idx] += 1.5
+    assert (C.double() - ref).abs().max().item() < 1e-4
+
+
+def test_gemm_split_k_atomic(cuda):
+    from fasthigashi import ops, _lib as L
+    gen = torch.Generator().manual_seed(11)
+    M, N, K = 64, 64, 5000
+    A = torch.randn(M, K, generator=gen).to(cuda)
+    B = torch.randn(K, N, generator=gen).to(cuda)
+    C = torch.zeros(M, N, device=cuda)
+    d = ops.make_desc(A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, K, 1, N, 1, N, 1,
+                      flags=L.GEMM_ATOMIC, ksplit=8)
+    ops.GemmGroup([d], L.F32).run()
+    ref = A.double() @ B.double()
+    assert (C.double() - ref).abs().max().item() < 1e-3
+
+
+def test_gemm_grouped(cuda):
+    from fasthigashi import ops, _lib as L
+    gen = torch.Generator().manual_seed(13)
+    mats = []
+    descs = []
+    for (M, N, K) in [(100, 32, 64), (257, 64, 40), (16, 16, 8)]:
+        A = torch.randn(M, K, generator=gen).to(cuda)
+        B = torch.randn(K, N, generator=gen).to(cuda)
+        C = torch.zeros(M, N, device=cuda)
+        mats.append((A, B, C))
+        descs.append(ops.make_desc(A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, K, 1, N, 1, N, 1))
+    ops.GemmGroup(descs, L.F32, shape=1).run()
+    for A, B, C in mats:
+        ref = A.double() @ B.double()
+        assert (C.double() - ref).abs().max().item() < 1e-4
+
+
+def test_gemm_bf16_out(cuda):
+    from fasthigashi import ops, _lib as L
+    gen = torch.Generator().manual_seed(17)
+    M, N, K = 96, 80, 128
+    A = torch.randn(M, K, generator=gen).to(torch.bfloat16).to(cuda)
+    B = torch.randn(K, N, generator=gen).to(torch.bfloat16).to(cuda)
+    C = torch.zeros(M, N, dtype=torch.bfloat16, device=cuda)
+    d = ops.make_desc(A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, K, 1, N, 1, N, 1,
+                      flags=L.GEMM_OUT_BF16)
+    ops.GemmGroup([d], L.BF16).run()
+    ref = (A.double() @ B.double()).to(torch.bfloat16)
+    assert (C.double() - ref.double()).abs().max().item() <= 0.02 * ref.double().abs().max().item()
+
+
+@pytest.mark.parametrize("adt,bdt,cdt", [(torch.float32, torch.float32, torch.float64),
+                                         (torch.float64, torch.float64, torch.float64),
+                                         (torch.float32, torch.float64, torch.float32)])
+def test_gemm64(cuda, adt, bdt, cdt):
+    from fasthigashi import ops
+    gen = torch.Generator().manual_seed(19)
+    A = torch.randn(3, 150, 77, generator=gen, dtype=torch.float64).to(adt).to(cuda)
+    B = torch.randn(3, 77, 131, generator=gen, dtype=torch.float64).to(bdt).to(cuda)
+    C = torch.zeros(3, 150, 131, dtype=cdt, device=cuda)
+    ops.gemm(A, B, C)
+    ref = A.double().cpu() @ B.double().cpu()
+    tol = 1e-12 if cdt == torch.float64 else 1e-5
+    assert (C.double().cpu() - ref).abs().max().item() < tol * ref.abs().max().item()
+
+
+def test_polar_matches_svd(cuda):
+    from fasthigashi import ops
+    rng = np.random.default_rng(0)
+    b, m, s = 6, 120, 40
+    U, _ = np.linalg.qr(rng.standard_normal((b, m, s)))
+    V, _ = np.linalg.qr(rng.standard_normal((b, s, s)))
+    S = np.exp(rng.uniform(np.log(1e-4), 0, size=(b, s)))     # kappa up to 1e4
+    Y = (U * S[:, None, :]) @ V.transpose(0, 2, 1)
+    Y32 = Y.astype(np.float32)
+    P, nuc = ops.polar(torch.from_numpy(Y32).to(cuda))
+    u, sv, vh = np.linalg.svd(Y32.astype(np.float64), full_matrices=False)
+    Pref = u @ vh
+    err = np.abs(P.cpu().numpy() - Pref).max()
+    assert err < 1e-5, err
+    assert np.allclose(nuc.cpu().numpy(), sv.sum(-1), rtol=1e-9)
+    PtP = np.einsum("bmi,bmj->bij", P.cpu().numpy().astype(np.float64), P.cpu().numpy().astype(np.float64))
+    assert np.abs(PtP - np.eye(s)).max() < 1e-5
+
+
+def test_polar_ill_conditioned(cuda):
+    from fasthigashi import ops
+    rng = np.random.default_rng(1)
+    b, m, s = 4, 300, 100
+    U, _ = np.linalg.qr(rng.standard_normal((b, m, s)))
+    V, _ = np.linalg.qr(rng.standard_normal((b, s, s)))
+    S = np.exp(rng.uniform(np.log(1e-6), 0, size=(b, s)))     # kappa ~ 1e6 (Gram 1e12)
+    Y = ((U * S[:, None, :]) @ V.transpose(0, 2, 1)).astype(np.float32)
+    P, nuc = ops.polar(torch.from_numpy(Y).to(cuda))
+    u, sv, vh = np.linalg.svd(Y.astype(np.float64), full_matrices=False)
+    Pref = u @ vh
+    err = np.abs(P.cpu().numpy() - Pref).max()
+    assert err < 1e-3, err
+    assert np.allclose(nuc.cpu().numpy(), sv.sum(-1), rtol=1e-7)
+
+
+def test_spd_inverse(cuda):
+    from fasthigashi import ops, _lib as L
+    rng = np.random.default_rng(2)
+    mats, outs, sys = [], [], []
+    for s in (5, 64, 150):
+        X = rng.standard_normal((s + 10, s))
+        G = torch.from_numpy(X.T @ X).to(cuda)
+        O = torch.zeros(s, s, dtype=torch.float64, device=cuda)
+        q = L.SpdSys()
+        q.G, q.out, q.s, q.ldg, q.ldo, q.eps = G.data_ptr(), O.data_ptr(), s, s, s, 1e-10
+        mats.append(G); outs.append(O); sys.append(q)
+    tab = ops._upload((L.SpdSys * 3)(*sys), cuda)
+    status = torch.zeros(1, dtype=torch.int32, device=cuda)
+    L.call("fh_spd_inverse", tab.data_ptr(), 3, 150, status.data_ptr(), ops.stream_handle())
+    for G, O in zip(mats, outs):
+        Gn = G.cpu().numpy() + 1e-10 * np.eye(G.shape[0])
+        ref = np.linalg.inv(Gn)
+        assert np.abs(O.cpu().numpy() - ref).max() < 1e-8 * np.abs(ref).max()
+    assert status.item() == 0
+
+
+def test_csr_sort_matches_oracle(cuda):
+    from fasthigashi import ops, _lib as L
+    from oracle import ref_np as R
+    rng = np.random.default_rng(3)
+    n_rows, n_cells = 97, 40
+    parts = []
+    for cell in range(n_cells):
+        k = rng.integers(0, 300)
+        r = np.sort(rng.integers(0, n_rows, k))
+        c = rng.integers(0, n_rows, k)
+        parts.append(np.stack([r, c, np.full(k, cell)]))
+    idx = np.concatenate(parts, axis=1).astype(np.int32)
+    val = rng.standard_normal(idx.shape[1]).astype(np.float32)
+    ridx, rval, rptr = R.sort_indices(idx, val, (n_rows, n_rows, n_cells))
+    nnz = idx.shape[1]
+    for chunk in (64, 256, 4096):
+        ws = L.lib().fh_csr_sort_workspace(nnz, n_rows, chunk)
+        wsb = torch.zeros(ws, dtype=torch.uint8, device=cuda)
+        di = torch.from_numpy(idx).to(cuda)
+        dv = torch.from_numpy(val).to(cuda)
+        oi = torch.zeros_like(di)
+        ov = torch.zeros_like(dv)
+        ptr = torch.zeros(n_rows + 1, dtype=torch.int64, device=cuda)
+        L.call("fh_csr_sort", di.data_ptr(), dv.data_ptr(), nnz, n_rows, chunk, oi.data_ptr(), ov.data_ptr(),
+               ptr.data_ptr(), wsb.data_ptr(), ws, ops.stream_handle())
+        assert np.array_equal(oi.cpu().numpy(), ridx)
+        assert np.array_equal(ov.cpu().numpy(), rval)
+        assert np.array_equal(ptr.cpu().numpy(), rptr)
