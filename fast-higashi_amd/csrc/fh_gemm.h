@@ -17,6 +17,7 @@ typedef struct FhGemmDesc {
   int64_t M, N, K;
   int64_t sAm, sAk, sBk, sBn, sCm, sCn;
   int64_t sAb, sBb, sCb;
+  int64_t sCk;            // FH_GEMM_SPLIT_PARTIAL: stride between the k-split partial outputs
   int32_t batch;
   int32_t flags;          // FH_GEMM_ATOMIC: C += alpha*AB with atomics (beta ignored)
   double alpha, beta, diag;
@@ -27,8 +28,9 @@ typedef struct FhGemmDesc {
   int32_t tiles_m, tiles_n;
 } FhGemmDesc;
 
-enum { FH_GEMM_ATOMIC = 1, FH_GEMM_OUT_BF16 = 2 };
-// bits 8.. of flags: k-split count (0 or 1 = no split; >1 requires FH_GEMM_ATOMIC)
+enum { FH_GEMM_ATOMIC = 1, FH_GEMM_OUT_BF16 = 2, FH_GEMM_SPLIT_PARTIAL = 4 };
+// bits 8.. of flags: k-split count (0 or 1 = no split; >1 requires FH_GEMM_ATOMIC or
+// FH_GEMM_SPLIT_PARTIAL, which writes split ks to C + ks*sCk for a fixed-order reduction)
 
 #ifdef __cplusplus
 }

@@ -255,8 +255,9 @@ __global__ __launch_bounds__(NT) void gemm_mfma_kernel(const FhGemmDesc* __restr
   float scale = (float)d.alpha;
   if (d.scale_a) scale *= d.scale_a[b];
   if (d.scale_b) scale *= d.scale_b[b];
-  float* C = (float*)d.C + b * d.sCb;
-  uint16_t* Ch = (uint16_t*)d.C + b * d.sCb;
+  const int64_t coff = b * d.sCb + ((d.flags & FH_GEMM_SPLIT_PARTIAL) ? ks * d.sCk : 0);
+  float* C = (float*)d.C + coff;
+  uint16_t* Ch = (uint16_t*)d.C + coff;
   const bool atomic = (d.flags & FH_GEMM_ATOMIC) != 0;
   const bool out_bf16 = (d.flags & FH_GEMM_OUT_BF16) != 0;
   const float beta = (float)d.beta, diag = (float)d.diag;
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(NT) void gemm_mfma_kernel(const FhGemmDesc* __restr
             atomicAdd(cp, v);
           } else {
             if (beta != 0.f) v += beta * *cp;
-            if (row == col) v += diag;
+            if (row == col && ks == 0) v += diag;
             *cp = v;
           }
         }
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restri
     }
   }
 
-  TC* C = (TC*)d.C + b * d.sCb;
+  TC* C = (TC*)d.C + b * d.sCb + ((d.flags & FH_GEMM_SPLIT_PARTIAL) ? ks * d.sCk : 0);
   const bool atomic = (d.flags & FH_GEMM_ATOMIC) != 0;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -377,7 +378,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restri
             atomicAdd(cp, (TC)v);
           } else {
             if (d.beta != 0.) v += d.beta * (double)*cp;
-            if (row == col) v += d.diag;
+            if (row == col && ks == 0) v += d.diag;
             *cp = (TC)v;
           }
         }
@@ -429,6 +430,7 @@ int launch_shape(int shape, bool ak, bool bk, const FhGemmDesc* dd, int n, const
     case 3: return launch_typed<T, 4, 5>(ak, bk, dd, n, single, tiles, st);   // 128 x 160
     case 4: return launch_typed<T, 2, 4>(ak, bk, dd, n, single, tiles, st);   //  64 x 128
     case 5: return launch_typed<T, 8, 4>(ak, bk, dd, n, single, tiles, st);   // 256 x 128
+    case 6: return launch_typed<T, 4, 8>(ak, bk, dd, n, single, tiles, st);   // 128 x 256
     default: fh_set_error("unknown gemm tile shape %d", shape); return FH_ERR_ARG;
   }
 }
@@ -436,8 +438,8 @@ int launch_shape(int shape, bool ak, bool bk, const FhGemmDesc* dd, int n, const
 }  // namespace
 
 extern "C" void fh_gemm_shape_dims(int shape, int* bm, int* bn) {
-  static const int dims[6][2] = {{128, 128}, {128, 64}, {64, 64}, {128, 160}, {64, 128}, {256, 128}};
-  if (shape < 0 || shape > 5) { *bm = *bn = 0; return; }
+  static const int dims[7][2] = {{128, 128}, {128, 64}, {64, 64}, {128, 160}, {64, 128}, {256, 128}, {128, 256}};
+  if (shape < 0 || shape > 6) { *bm = *bn = 0; return; }
   *bm = dims[shape][0];
   *bn = dims[shape][1];
 }

@@ -204,6 +204,16 @@ __global__ void sweep_errors_kernel(const double* tnorm, const double* core, con
   }
 }
 
+// out[e] = sum_b in[b * stride + e]  (fixed order), fp32 or fp64
+template <typename T>
+__global__ void batch_sum_kernel(const T* in, int64_t nb, int64_t stride, int64_t count, T* out) {
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < count; e += (int64_t)gridDim.x * NT) {
+    double acc = 0.;
+    for (int64_t b = 0; b < nb; ++b) acc += (double)in[b * stride + e];
+    out[e] = (T)acc;
+  }
+}
+
 // CP-ALS inner-iteration test (parafac_integrative.py:80-108) from closed forms:
 //   loss = ||T||^2 - 2 <M_D, D> + ||[[A,B,D]]||^2 ; wdiff = (prev - loss) / prev; stop if wdiff < 1e-5
 // state: [0] = previous loss, [1] = have previous; flag: 1 once stopped.
@@ -326,6 +336,18 @@ int fh_dot2d(const void* X, int xdt, int64_t ldx, const void* Y, int ydt, int64_
 int fh_sweep_errors(const double* tnorm, const double* core, const double* xU, const double* xV, int K,
                     double* hist, hipStream_t st) {
   hipLaunchKernelGGL(sweep_errors_kernel, dim3(1), dim3(64), 0, st, tnorm, core, xU, xV, K, hist);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_batch_sum(const void* in, int dt, int64_t nb, int64_t stride, int64_t count, void* out, hipStream_t st) {
+  if (count <= 0) return FH_OK;
+  if (dt == FH_F64)
+    hipLaunchKernelGGL((batch_sum_kernel<double>), dim3(grid_for(count)), dim3(NT), 0, st, (const double*)in, nb,
+                       stride, count, (double*)out);
+  else
+    hipLaunchKernelGGL((batch_sum_kernel<float>), dim3(grid_for(count)), dim3(NT), 0, st, (const float*)in, nb,
+                       stride, count, (float*)out);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_native", "libfh_hip.so")
 
 F32, BF16, F64, FP8 = 0, 1, 2, 3
-GEMM_ATOMIC, GEMM_OUT_BF16 = 1, 2
+GEMM_ATOMIC, GEMM_OUT_BF16, GEMM_SPLIT_PARTIAL = 1, 2, 4
 
 
 class GemmDesc(Structure):
@@ -24,7 +24,7 @@ class GemmDesc(Structure):
         ("M", c_int64), ("N", c_int64), ("K", c_int64),
         ("sAm", c_int64), ("sAk", c_int64), ("sBk", c_int64), ("sBn", c_int64),
         ("sCm", c_int64), ("sCn", c_int64),
-        ("sAb", c_int64), ("sBb", c_int64), ("sCb", c_int64),
+        ("sAb", c_int64), ("sBb", c_int64), ("sCb", c_int64), ("sCk", c_int64),
         ("batch", c_int32), ("flags", c_int32),
         ("alpha", c_double), ("beta", c_double), ("diag", c_double),
         ("scale_a", c_void_p), ("scale_b", c_void_p), ("skip", c_void_p),
@@ -94,6 +94,7 @@ _SIGS = {
     "fh_dot2d": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p,
                          c_void_p]),
     "fh_sweep_errors": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "fh_batch_sum": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "fh_cpals_check": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fh_csr_sort_workspace": (c_int64, [c_int64, c_int32, c_int64]),
     "fh_csr_sort": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_void_p, c_void_p, c_void_p,

@@ -1,0 +1,544 @@
+"""Device-resident PARAFAC2 ALS sweep on gfx950 (the hot path of Fast_Higashi_core.fit).
+
+Restates one iteration of the fit loop (parafac2_intergrative.py:635-737):
+
+  pass 1   C_c = M D_c;  temp_i = X_i C_c diag(A_i) B_c^T          (:336, :374-383)
+  polar    P_i = polar(temp_i), x_U[c] = sum_i ||temp_i||_*          (:396-417)
+  pass 2   SVD_term = sum_c D_c sum_i diag(A_i) B_c^T P_i^T X_i      (:420-478)
+  meta     M = polar(SVD_term^T), x_V = ||SVD_term||_*               (:481-486)
+  pass 3   T_c[i] = P_i^T X_i M                                      (:488-529)
+  errors   err_U, err_V with the previous core norm                  (:667-668)
+  CP-ALS   on every T_c, then the new core norms                     (:674-706)
+
+The association differs from the reference where it saves work without changing the
+result: X_i C_c is one GEMM per chromosome (C is shared by all its bins), T_hat_i =
+P_i^T X_i is computed once and reused by pass 2 and pass 3, pass 2 folds
+diag(A_i) B^T into a Khatri-Rao operand L_c, and the core norms use the closed form
+1^T (A^T A o B^T B o D^T D) 1.
+
+Data layout in HBM (cells innermost, padded to Npad = multiple of 64):
+  X       (sum_c W_c) x Npad   per chromosome its tiles, per tile rows x width slices
+  T_hat   (sum_c n_c sp_c) x Npad
+  A_c     n_c x sp_c,  B_c s_c x sp_c,  D_all R x sum sp_c,  M Npad x R
+(sp_c = s_c rounded up to 8; padding rows / columns are zero and stay zero).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import ops
+from .ops import make_desc, rup
+
+F32, BF16, F64 = L.F32, L.BF16, L.F64
+
+
+@dataclass
+class TileSpec:
+    start: int
+    rows: int
+    width: int
+    col_start: int
+    local_start: int
+    shift: int
+
+
+@dataclass
+class ChromLayout:
+    name: str
+    n: int
+    s: int
+    sp: int
+    tiles: List[TileSpec]
+    W: int = 0
+    xrow: int = 0          # first row of the chromosome in X
+    trow: List[int] = field(default_factory=list)   # first row of each tile, relative to xrow
+    aoff: int = 0          # element offset of A_c in A_all
+    boff: int = 0          # element offset of B_c in B_all
+    doff: int = 0          # first column of D_c in D_all (and of Z_c in Zt)
+    toff: int = 0          # first row of the chromosome in T_hat (n * sp rows)
+    yoff: int = 0          # element offset in Y / temp / P (W x sp)
+    ttoff: int = 0         # element offset in T_tilde (n*sp x R)
+    loff: int = 0          # element offset in L (s x n*sp)
+
+
+def shape_for_n(N: int) -> int:
+    if N <= 64:
+        return 1
+    if N <= 128:
+        return 0
+    if N <= 160:
+        return 3
+    return 6
+
+
+def shape_for_m(M: int) -> int:
+    """tile shape whose BM covers M rows (pass-2 GEMM: M = s, streamed operand X)."""
+    return 0 if M <= 128 else 5
+
+
+class MultiGemm:
+    """A set of descriptors split into one grouped launch per tile shape."""
+
+    def __init__(self, descs, dtype, shape_of, device):
+        by = {}
+        for d in descs:
+            by.setdefault(shape_of(d), []).append(d)
+        self.groups = [ops.GemmGroup(v, dtype, k, device) for k, v in sorted(by.items())]
+
+    def run(self, stream=None):
+        for g in self.groups:
+            g.run(stream)
+
+
+class SweepEngine:
+    """ALS sweeps over a fixed dataset resident in HBM.
+
+    tiles: per chromosome a list of TileSpec (the Chrom_Dataset tiling); sizes: s_c.
+    dtype: 'fp32' (plain FP32 path) or 'bf16' (X and the X-streaming GEMM operands in
+    bf16, fp32 accumulation).  Polar factors, Gram solves and the error terms are FP64.
+    """
+
+    def __init__(self, tiles: Sequence[Sequence[TileSpec]], n_bins: Sequence[int], sizes: Sequence[int],
+                 n_cells: int, rank: int, dtype: str = "fp32", device="cuda", names=None,
+                 ns_iter: int = 48, ns_tol: float = 1e-24, meta_ksplit: int = 32, n_iter_max_hist: int = 512):
+        assert dtype in ("fp32", "bf16")
+        self.dtype = dtype
+        self.xdt = F32 if dtype == "fp32" else BF16
+        self.tdtype = torch.float32 if dtype == "fp32" else torch.bfloat16
+        self.device = torch.device(device)
+        self.N = int(n_cells)
+        self.Npad = rup(self.N, 64)
+        self.R = int(rank)
+        assert self.R % 8 == 0, "rank must be a multiple of 8"
+        self.K = len(tiles)
+        self.ns_iter = ns_iter
+        self.ns_tol = ns_tol
+        names = names or ["chr%d" % (i + 1) for i in range(self.K)]
+        self.chroms: List[ChromLayout] = []
+        xrow = aoff = boff = doff = toff = yoff = ttoff = loff = 0
+        for c in range(self.K):
+            s = int(sizes[c])
+            assert 1 <= s <= self.R
+            sp = rup(s, 8)
+            cl = ChromLayout(names[c], int(n_bins[c]), s, sp, list(tiles[c]))
+            tr = 0
+            for t in cl.tiles:
+                cl.trow.append(tr)
+                tr += t.rows * t.width
+            cl.W = tr
+            cl.xrow, cl.aoff, cl.boff, cl.doff = xrow, aoff, boff, doff
+            cl.toff, cl.yoff, cl.ttoff, cl.loff = toff, yoff, ttoff, loff
+            xrow += cl.W
+            aoff += cl.n * sp
+            boff += s * sp
+            doff += sp
+            toff += cl.n * sp
+            yoff += cl.W * sp
+            ttoff += cl.n * sp * self.R
+            loff += s * cl.n * sp
+            self.chroms.append(cl)
+        self.Wtot, self.Dld, self.Ttot = xrow, doff, toff
+        dev = self.device
+        z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
+        # data and factors
+        self.X = z(self.Wtot, self.Npad, dt=self.tdtype)
+        self.A_all = z(aoff)
+        self.B_all = z(boff)
+        self.D_all = z(self.R, self.Dld)
+        self.M = z(self.Npad, self.R)
+        # work buffers
+        self.Ct = z(self.Dld, self.Npad, dt=self.tdtype)
+        self.Y = z(yoff)
+        self.temp = z(yoff)
+        self.P = z(yoff)
+        self.P16 = z(yoff, dt=torch.bfloat16) if dtype == "bf16" else None
+        self.That = z(self.Ttot, self.Npad, dt=self.tdtype)
+        self.Lk = z(loff, dt=self.tdtype)
+        self.Zt = z(self.Npad, self.Dld)
+        self.meta_ksplit = meta_ksplit
+        self.SVDt = z(self.Npad, self.R)
+        self.GMpart = z(meta_ksplit * self.R * self.R, dt=torch.float64)
+        self.M16 = z(self.Npad, self.R, dt=torch.bfloat16) if dtype == "bf16" else None
+        self.Ttil = z(ttoff)
+        # polar batches: per-bin (one group per chromosome) and the meta-embedding
+        self.pb = ops.PolarBatch([c.s for c in self.chroms], [c.n for c in self.chroms], ns_iter, ns_tol, dev)
+        self.pbm = ops.PolarBatch([self.R], [1], ns_iter, ns_tol, dev)
+        self.zbuf = ns_iter & 1       # buffer index holding the result (fixed launch count)
+        # CP-ALS per-chromosome buffers
+        smax = max(c.s for c in self.chroms)
+        spmax = max(c.sp for c in self.chroms)
+        nspmax = max(c.n * c.sp for c in self.chroms)
+        self.W1 = [z(c.n * c.sp * c.sp) for c in self.chroms]
+        self.Gr = [z(4, c.s, c.sp, dt=torch.float64) for c in self.chroms]   # GA, GB, GD, H
+        self.Hinv = [z(c.s, c.sp, dt=torch.float64) for c in self.chroms]
+        self.MF = [z(max(c.n, self.R) * c.sp) for c in self.chroms]         # MTTKRP results
+        self.KRf = [z(c.s * c.n * c.sp) for c in self.chroms]               # fp32 Khatri-Rao for M_D
+        self.mdsplit = [max(1, min(32, (c.n * c.sp) // 2048)) for c in self.chroms]
+        self.MDpart = [z(k * self.R * c.sp) for k, c in zip(self.mdsplit, self.chroms)]
+        self.flags = torch.zeros(self.K, dtype=torch.int32, device=dev)
+        self.cp_state = z(self.K, 2, dt=torch.float64)
+        self.cp_scal = z(self.K, 4, dt=torch.float64)       # tnorm2, inner, xhat2, spare
+        self.spd_status = torch.zeros(1, dtype=torch.int32, device=dev)
+        # scalars
+        self.tnorm = z(self.K, dt=torch.float64)
+        self.xU = z(self.K, dt=torch.float64)
+        self.core = [z(self.K, dt=torch.float64), z(self.K, dt=torch.float64)]
+        self.core_cur = 0
+        self.hist_w = 3 + 3 * self.K
+        self.hist = z(n_iter_max_hist, self.hist_w, dt=torch.float64)
+        self.seg = torch.tensor(np.concatenate([[0], np.cumsum([c.n for c in self.chroms])]),
+                                dtype=torch.int64, device=dev)
+        self.iteration = 0
+        self.n_iter_parafac = 1
+        self._build()
+
+    # ------------------------------------------------------------------ views
+    def A_view(self, c: int) -> torch.Tensor:
+        cl = self.chroms[c]
+        return self.A_all[cl.aoff: cl.aoff + cl.n * cl.sp].view(cl.n, cl.sp)
+
+    def B_view(self, c: int) -> torch.Tensor:
+        cl = self.chroms[c]
+        return self.B_all[cl.boff: cl.boff + cl.s * cl.sp].view(cl.s, cl.sp)
+
+    def D_view(self, c: int) -> torch.Tensor:
+        cl = self.chroms[c]
+        return self.D_all[:, cl.doff: cl.doff + cl.sp]
+
+    def X_tile(self, c: int, t: int) -> torch.Tensor:
+        cl = self.chroms[c]
+        tl = cl.tiles[t]
+        r0 = cl.xrow + cl.trow[t]
+        return self.X[r0: r0 + tl.rows * tl.width].view(tl.rows, tl.width, self.Npad)
+
+    def P_tile(self, c: int, t: int) -> torch.Tensor:
+        cl = self.chroms[c]
+        tl = cl.tiles[t]
+        o = cl.yoff + cl.trow[t] * cl.sp
+        return self.P[o: o + tl.rows * tl.width * cl.sp].view(tl.rows, tl.width, cl.sp)
+
+    def T_view(self, c: int) -> torch.Tensor:
+        cl = self.chroms[c]
+        return self.Ttil[cl.ttoff: cl.ttoff + cl.n * cl.sp * self.R].view(cl.n, cl.sp, self.R)
+
+    # ------------------------------------------------------------------ setup
+    def set_factors(self, A: Sequence[np.ndarray], B: Sequence[np.ndarray], D: Sequence[np.ndarray],
+                    M: np.ndarray):
+        """Load initial factors (A_c n x s, B_c s x s, D_c R x s, M N x R)."""
+        self.A_all.zero_(); self.B_all.zero_(); self.D_all.zero_(); self.M.zero_()
+        for c, cl in enumerate(self.chroms):
+            self.A_view(c)[:, :cl.s] = torch.as_tensor(np.asarray(A[c], dtype=np.float32), device=self.device)
+            self.B_view(c)[:, :cl.s] = torch.as_tensor(np.asarray(B[c], dtype=np.float32), device=self.device)
+            self.D_view(c)[:, :cl.s] = torch.as_tensor(np.asarray(D[c], dtype=np.float32), device=self.device)
+        self.M[:self.N] = torch.as_tensor(np.asarray(M, dtype=np.float32), device=self.device)
+        self._core_norms(self.core[self.core_cur])
+
+    def get_factors(self):
+        A = [self.A_view(c)[:, :cl.s].cpu().numpy().copy() for c, cl in enumerate(self.chroms)]
+        B = [self.B_view(c)[:, :cl.s].cpu().numpy().copy() for c, cl in enumerate(self.chroms)]
+        D = [self.D_view(c)[:, :cl.s].cpu().numpy().copy() for c, cl in enumerate(self.chroms)]
+        M = self.M[:self.N].cpu().numpy().copy()
+        return A, B, D, M
+
+    def set_X_tiles(self, tiles_per_chrom):
+        """Load dense post-RWR tiles (rows x w x N arrays, cells last) into X."""
+        for c, cl in enumerate(self.chroms):
+            for t, tl in enumerate(cl.tiles):
+                arr = torch.as_tensor(np.asarray(tiles_per_chrom[c][t]), device=self.device)
+                self.X_tile(c, t)[:, :, :self.N] = arr.to(self.tdtype)
+        self.finish_X()
+
+    def finish_X(self):
+        """Squared norms of each chromosome's X (the reference's first-sweep ||X||^2)."""
+        rpb = 64
+        for c, cl in enumerate(self.chroms):
+            X = self.X[cl.xrow: cl.xrow + cl.W]
+            nb = -(-cl.W // rpb)
+            part = torch.zeros(nb, dtype=torch.float64, device=self.device)
+            L.call("fh_sumsq_partial", X.data_ptr(), self.xdt, cl.W, self.Npad, self.Npad, rpb, part.data_ptr(),
+                   ops.stream_handle())
+            off = torch.tensor([0, nb], dtype=torch.int64, device=self.device)
+            L.call("fh_segment_sum", part.data_ptr(), off.data_ptr(), 1, self.tnorm.data_ptr() + 8 * c, 0,
+                   ops.stream_handle())
+
+    # ------------------------------------------------------------------ plan
+    def _build(self):
+        dev = self.device
+        R, Np, Dld = self.R, self.Npad, self.Dld
+        xe = 2 if self.dtype == "bf16" else 4
+        out16 = L.GEMM_OUT_BF16 if self.dtype == "bf16" else 0
+        Xp, Ctp, Thp, Lp = self.X.data_ptr(), self.Ct.data_ptr(), self.That.data_ptr(), self.Lk.data_ptr()
+        f4 = lambda t, o: t.data_ptr() + 4 * o
+        # (1) Ct_c = (M D_c)^T, computed as M D_c with a transposed store
+        d1 = []
+        for cl in self.chroms:
+            d1.append(make_desc(self.M.data_ptr(), f4(self.D_all, cl.doff), Ctp + xe * cl.doff * Np,
+                                Np, cl.s, R, R, 1, Dld, 1, 1, Np, flags=out16))
+        self.g_ct = MultiGemm(d1, F32, lambda d: shape_for_n(d.N), dev)
+        # (2) Y_c = X_c C_c
+        d2 = []
+        for cl in self.chroms:
+            d2.append(make_desc(Xp + xe * cl.xrow * Np, Ctp + xe * cl.doff * Np, f4(self.Y, cl.yoff),
+                                cl.W, cl.s, Np, Np, 1, 1, Np, cl.sp, 1))
+        self.g_y = MultiGemm(d2, self.xdt, lambda d: shape_for_n(d.N), dev)
+        # (3) Y rows *= A[bin]
+        rs = []
+        for c, cl in enumerate(self.chroms):
+            for t, tl in enumerate(cl.tiles):
+                q = L.RowScale()
+                q.Y = f4(self.Y, cl.yoff + cl.trow[t] * cl.sp)
+                q.A = f4(self.A_all, cl.aoff + tl.start * cl.sp)
+                q.nrows, q.w, q.s, q.ldy, q.lda = tl.rows * tl.width, tl.width, cl.s, cl.sp, cl.sp
+                rs.append(q)
+        self.rs_tab = ops._upload((L.RowScale * len(rs))(*rs), dev)
+        self.rs_n = len(rs)
+        self.rs_max = max(q.nrows * q.s for q in rs)
+        # (4) temp_c = Y'_c B_c^T
+        d4 = []
+        for cl in self.chroms:
+            d4.append(make_desc(f4(self.Y, cl.yoff), f4(self.B_all, cl.boff), f4(self.temp, cl.yoff),
+                                cl.W, cl.s, cl.s, cl.sp, 1, 1, cl.sp, cl.sp, 1))
+        self.g_temp = MultiGemm(d4, F32, lambda d: shape_for_n(d.N), dev)
+        # (5) per-bin Grams G_i = temp_i^T temp_i (FP64)
+        d5 = []
+        for c, cl in enumerate(self.chroms):
+            G = self.pb.G(c)
+            for t, tl in enumerate(cl.tiles):
+                a = f4(self.temp, cl.yoff + cl.trow[t] * cl.sp)
+                d5.append(make_desc(a, a, G.data_ptr() + 8 * tl.start * G.stride(0), cl.s, cl.s, tl.width,
+                                    1, cl.sp, cl.sp, 1, G.stride(1), 1, tl.rows, tl.width * cl.sp,
+                                    tl.width * cl.sp, G.stride(0)))
+        self.g_gram = ops.Gemm64Group(d5, F32, F32, F64, dev)
+        # (7) P_i = temp_i G_i^{-1/2}
+        self.pb.result = self.zbuf
+        d7 = []
+        for c, cl in enumerate(self.chroms):
+            Z = self.pb.Zinv(c)
+            for t, tl in enumerate(cl.tiles):
+                o = cl.yoff + cl.trow[t] * cl.sp
+                d7.append(make_desc(f4(self.temp, o), Z.data_ptr() + 8 * tl.start * Z.stride(0), f4(self.P, o),
+                                    tl.width, cl.s, cl.s, cl.sp, 1, Z.stride(1), 1, cl.sp, 1, tl.rows,
+                                    tl.width * cl.sp, Z.stride(0), tl.width * cl.sp))
+        self.g_p = ops.Gemm64Group(d7, F32, F64, F32, dev)
+        # (8) T_hat_i = P_i^T X_i
+        Pp = self.P16.data_ptr() if self.dtype == "bf16" else self.P.data_ptr()
+        d8 = []
+        for c, cl in enumerate(self.chroms):
+            for t, tl in enumerate(cl.tiles):
+                o = cl.yoff + cl.trow[t] * cl.sp
+                d8.append(make_desc(Pp + xe * o, Xp + xe * (cl.xrow + cl.trow[t]) * Np,
+                                    Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
+                                    cl.s, Np, tl.width, 1, cl.sp, Np, 1, Np, 1, tl.rows, tl.width * cl.sp,
+                                    tl.width * Np, cl.sp * Np, flags=out16))
+        self.g_that = MultiGemm(d8, self.xdt, lambda d: shape_for_m(d.M), dev)
+        # (10) Zt_c = T_hat_c^T L_c^T
+        d10 = []
+        for cl in self.chroms:
+            d10.append(make_desc(Thp + xe * cl.toff * Np, Lp + xe * cl.loff, f4(self.Zt, cl.doff),
+                                 Np, cl.s, cl.n * cl.sp, 1, Np, 1, cl.n * cl.sp, Dld, 1))
+        self.g_z = MultiGemm(d10, self.xdt, lambda d: shape_for_n(d.N), dev)
+        # (11) SVD_term^T = Zt D_all^T
+        d11 = make_desc(self.Zt.data_ptr(), self.D_all.data_ptr(), self.SVDt.data_ptr(), Np, R, Dld,
+                        Dld, 1, 1, Dld, R, 1)
+        self.g_svd = ops.GemmGroup([d11], F32, 6 if R > 160 else shape_for_n(R), dev)
+        # (12) G_M = SVD_term SVD_term^T in k-split partials (FP64)
+        Gm = self.pbm.G(0)
+        d12 = make_desc(self.SVDt.data_ptr(), self.SVDt.data_ptr(), self.GMpart.data_ptr(), R, R, Np,
+                        1, R, R, 1, R, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=self.meta_ksplit, sCk=R * R)
+        self.g_gm = ops.Gemm64Group([d12], F32, F32, F64, dev)
+        self.Gm = Gm
+        # (13) M = SVD_term^T G_M^{-1/2}
+        self.pbm.result = self.zbuf
+        Zm = self.pbm.Zinv(0)
+        d13 = make_desc(self.SVDt.data_ptr(), Zm.data_ptr(), self.M.data_ptr(), Np, R, R, R, 1,
+                        Zm.stride(1), 1, R, 1)
+        self.g_m = ops.Gemm64Group([d13], F32, F64, F32, dev)
+        # (14) T_tilde_c = T_hat_c M
+        Mp = self.M16.data_ptr() if self.dtype == "bf16" else self.M.data_ptr()
+        d14 = []
+        for cl in self.chroms:
+            d14.append(make_desc(Thp + xe * cl.toff * Np, Mp, f4(self.Ttil, cl.ttoff), cl.n * cl.sp, R, Np,
+                                 Np, 1, R, 1, R, 1))
+        self.g_ttil = MultiGemm(d14, self.xdt, lambda d: 6 if d.N > 160 else shape_for_n(d.N), dev)
+        self._build_cpals()
+
+    def _build_cpals(self):
+        dev, R, Dld = self.device, self.R, self.Dld
+        f4 = lambda t, o=0: t.data_ptr() + 4 * o
+        self.cp = []
+        for c, cl in enumerate(self.chroms):
+            n, s, sp = cl.n, cl.s, cl.sp
+            skip = self.flags.data_ptr() + 4 * c
+            Ap, Bp, Dp = f4(self.A_all, cl.aoff), f4(self.B_all, cl.boff), f4(self.D_all, cl.doff)
+            Gr = self.Gr[c]
+            gp = [Gr[i].data_ptr() for i in range(4)]
+            e = {}
+            # W1 = T (n sp x R) . D (R x s)
+            e["w1"] = ops.GemmGroup([make_desc(f4(self.Ttil, cl.ttoff), Dp, f4(self.W1[c]), n * sp, s, R,
+                                               R, 1, Dld, 1, sp, 1)], F32, shape_for_n(s), dev)
+            # Grams (FP64):  F^T F for F = A (n x s), B (s x s), D (R x s)
+            def gram(F, rows, ld, out):
+                return make_desc(F, F, out, s, s, rows, 1, ld, ld, 1, sp, 1)
+            e["gA"] = ops.Gemm64Group([gram(Ap, n, sp, gp[0])], F32, F32, F64, dev)
+            e["gB"] = ops.Gemm64Group([gram(Bp, s, sp, gp[1])], F32, F32, F64, dev)
+            e["gD"] = ops.Gemm64Group([gram(Dp, R, Dld, gp[2])], F32, F32, F64, dev)
+            q = L.SpdSys()
+            q.G, q.out, q.s, q.ldg, q.ldo, q.eps = gp[3], self.Hinv[c].data_ptr(), s, sp, sp, 1e-10
+            e["spd"] = ops._upload((L.SpdSys * 1)(q), dev)
+            MF = f4(self.MF[c])
+            Hi = self.Hinv[c].data_ptr()
+            # factor updates  F_new = MTTKRP (G + 1e-10 I)^{-1}   (skipped once the inner loop stopped)
+            e["uA"] = ops.Gemm64Group([make_desc(MF, Hi, Ap, n, s, s, sp, 1, 1, sp, sp, 1, skip=skip)],
+                                      F32, F64, F32, dev)
+            e["uB"] = ops.Gemm64Group([make_desc(MF, Hi, Bp, s, s, s, sp, 1, 1, sp, sp, 1, skip=skip)],
+                                      F32, F64, F32, dev)
+            e["uD"] = ops.Gemm64Group([make_desc(MF, Hi, Dp, R, s, s, sp, 1, 1, sp, Dld, 1, skip=skip)],
+                                      F32, F64, F32, dev)
+            # M_D = T^T KR  (R x s), K = n sp, split-k partials
+            ks = self.mdsplit[c]
+            e["md"] = ops.GemmGroup([make_desc(f4(self.Ttil, cl.ttoff), f4(self.KRf[c]), f4(self.MDpart[c]),
+                                               R, s, n * sp, 1, R, 1, n * sp, sp, 1,
+                                               flags=L.GEMM_SPLIT_PARTIAL if ks > 1 else 0, ksplit=ks,
+                                               sCk=R * sp)], F32, 6 if s > 160 else shape_for_n(s), dev)
+            self.cp.append(e)
+
+    # ------------------------------------------------------------------ sweep
+    def _core_norms(self, out: torch.Tensor, stream=None):
+        st = ops.stream_handle(stream)
+        for c, cl in enumerate(self.chroms):
+            e = self.cp[c]
+            e["gA"].run(stream); e["gB"].run(stream); e["gD"].run(stream)
+            G = self.Gr[c]
+            L.call("fh_sum3", G[0].data_ptr(), G[1].data_ptr(), G[2].data_ptr(), cl.s, cl.sp,
+                   out.data_ptr() + 8 * c, st)
+
+    def sweep(self, n_iter_parafac: Optional[int] = None, stream=None):
+        """One ALS iteration; the errors land in self.hist[self.iteration] (device)."""
+        st = ops.stream_handle(stream)
+        it = self.iteration
+        if n_iter_parafac is None:
+            if it % 10 == 0 and it > 0 and self.n_iter_parafac < 10:       # :636-637
+                self.n_iter_parafac += 1
+            n_iter_parafac = self.n_iter_parafac
+        # pass 1
+        self.g_ct.run(stream)
+        self.g_y.run(stream)
+        L.call("fh_scale_rows", self.rs_tab.data_ptr(), self.rs_n, self.rs_max, st)
+        self.g_temp.run(stream)
+        self.g_gram.run(stream)
+        self.pb.solve(stream)
+        self.g_p.run(stream)
+        L.call("fh_segment_sum", self.pb.nuc.data_ptr(), self.seg.data_ptr(), self.K, self.xU.data_ptr(), 0, st)
+        if self.dtype == "bf16":
+            L.call("fh_copy2d", self.P.data_ptr(), F32, self.P.numel(), 1, self.P16.data_ptr(), BF16,
+                   self.P.numel(), 1, self.P.numel(), 1.0, st)
+        # pass 2
+        self.g_that.run(stream)
+        for c, cl in enumerate(self.chroms):
+            L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,
+                   cl.sp, self.Lk.data_ptr() + self.Lk.element_size() * cl.loff, self.xdt, st)
+        self.g_z.run(stream)
+        self.g_svd.run(stream)
+        # meta-embedding polar
+        self.g_gm.run(stream)
+        L.call("fh_batch_sum", self.GMpart.data_ptr(), F64, self.meta_ksplit, self.R * self.R, self.R * self.R,
+               self.Gm.data_ptr(), st)
+        self.pbm.solve(stream)
+        self.g_m.run(stream)
+        if self.dtype == "bf16":
+            L.call("fh_copy2d", self.M.data_ptr(), F32, self.R, 1, self.M16.data_ptr(), BF16, self.R,
+                   self.Npad, self.R, 1.0, st)
+        # pass 3
+        self.g_ttil.run(stream)
+        # errors with the previous core norm (:667-668)
+        prev = self.core[self.core_cur]
+        L.call("fh_sweep_errors", self.tnorm.data_ptr(), prev.data_ptr(), self.xU.data_ptr(),
+               self.pbm.nuc.data_ptr(), self.K, self.hist.data_ptr() + 8 * self.hist_w * it, st)
+        # CP-ALS on every T_c, then the new core norm (:674-706)
+        self._cpals(n_iter_parafac, stream)
+        self.core_cur ^= 1
+        self._core_norms(self.core[self.core_cur], stream)
+        self.iteration += 1
+
+    def _cpals(self, n_iter: int, stream=None):
+        st = ops.stream_handle(stream)
+        ops.fill_(self.flags, 0, stream)
+        ops.fill_(self.cp_state, 0.0, stream)
+        for c, cl in enumerate(self.chroms):
+            T = self.T_view(c)
+            L.call("fh_dot2d", T.data_ptr(), F32, self.R, T.data_ptr(), F32, self.R, cl.n * cl.sp, self.R,
+                   self.cp_scal.data_ptr() + 8 * (4 * c), st)
+        for c, cl in enumerate(self.chroms):
+            Ap, Bp, Dp = self.A_view(c), self.B_view(c), self.D_view(c)
+            L.call("fh_balance_norm", Ap.data_ptr(), cl.sp, cl.n, Bp.data_ptr(), cl.sp, Dp.data_ptr(), self.Dld,
+                   self.R, cl.s, None, st)                                          # :65
+        for _ in range(n_iter):
+            for c, cl in enumerate(self.chroms):
+                self._cpals_iter(c, st, stream)
+
+    def _cpals_iter(self, c: int, st, stream):
+        cl = self.chroms[c]
+        e = self.cp[c]
+        n, s, sp, R = cl.n, cl.s, cl.sp, self.R
+        G = self.Gr[c]
+        g = [G[i].data_ptr() for i in range(4)]
+        Ap, Bp, Dp = self.A_view(c).data_ptr(), self.B_view(c).data_ptr(), self.D_view(c).data_ptr()
+        flag = self.flags.data_ptr() + 4 * c
+        MF = self.MF[c].data_ptr()
+        e["w1"].run(stream)                                     # W1 = T x_3 D
+        e["gB"].run(stream); e["gD"].run(stream)
+        # mode A
+        L.call("fh_hadamard", g[1], g[2], None, s, sp, g[3], st)
+        L.call("fh_spd_inverse", e["spd"].data_ptr(), 1, s, self.spd_status.data_ptr(), st)
+        L.call("fh_mttkrp_reduce", self.W1[c].data_ptr(), sp, Bp, sp, n, s, sp, 0, MF, sp, st)
+        e["uA"].run(stream)
+        # mode B
+        e["gA"].run(stream)
+        L.call("fh_hadamard", g[0], g[2], None, s, sp, g[3], st)
+        L.call("fh_spd_inverse", e["spd"].data_ptr(), 1, s, self.spd_status.data_ptr(), st)
+        L.call("fh_mttkrp_reduce", self.W1[c].data_ptr(), sp, Ap, sp, n, s, sp, 1, MF, sp, st)
+        e["uB"].run(stream)
+        # mode D
+        e["gB"].run(stream)
+        L.call("fh_hadamard", g[0], g[1], None, s, sp, g[3], st)
+        L.call("fh_spd_inverse", e["spd"].data_ptr(), 1, s, self.spd_status.data_ptr(), st)
+        L.call("fh_krp", Ap, sp, Bp, sp, n, s, sp, self.KRf[c].data_ptr(), F32, st)
+        e["md"].run(stream)
+        ks = self.mdsplit[c]
+        if ks > 1:
+            L.call("fh_batch_sum", self.MDpart[c].data_ptr(), F32, ks, R * sp, R * sp, MF, st)
+            mdp = MF
+        else:
+            mdp = self.MDpart[c].data_ptr()
+        if ks <= 1:
+            L.call("fh_copy2d", mdp, F32, sp, 1, MF, F32, sp, R, sp, 1.0, st)
+        e["uD"].run(stream)
+        # stopping test from closed forms (:80-107), then balance unless stopped (:108)
+        e["gD"].run(stream)
+        sc = self.cp_scal.data_ptr() + 8 * (4 * c)
+        L.call("fh_dot2d", MF, F32, sp, Dp, F32, self.Dld, R, s, sc + 8, st)
+        L.call("fh_sum3", g[0], g[1], g[2], s, sp, sc + 16, st)
+        L.call("fh_cpals_check", sc, sc + 8, sc + 16, self.cp_state.data_ptr() + 16 * c, flag, st)
+        L.call("fh_balance_norm", Ap, sp, n, Bp, sp, Dp, self.Dld, R, s, flag, st)
+
+    # ------------------------------------------------------------------ results
+    def history(self, n: Optional[int] = None):
+        n = self.iteration if n is None else n
+        h = self.hist[:n].cpu().numpy()
+        out = dict(rec_error=h[:, 0], err_V=h[:, 1], x_V=h[:, 2],
+                   rec_error_by_chrom=h[:, 3::3], x_U=h[:, 4::3], core_norm=h[:, 5::3])
+        return out
+
+    def projections(self):
+        """P_i per chromosome and tile as (rows, w, s) fp32 arrays."""
+        out = []
+        for c, cl in enumerate(self.chroms):
+            out.append([self.P_tile(c, t)[:, :, :cl.s].cpu().numpy().copy() for t in range(len(cl.tiles))])
+        return out

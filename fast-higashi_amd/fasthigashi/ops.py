@@ -18,7 +18,8 @@ from . import _lib as L
 DT_CODE = {torch.float32: L.F32, torch.bfloat16: L.BF16, torch.float64: L.F64, torch.uint8: L.FP8}
 
 # tile shapes of the MFMA kernel: index -> (BM, BN)
-SHAPES = {0: (128, 128), 1: (128, 64), 2: (64, 64), 3: (128, 160), 4: (64, 128), 5: (256, 128)}
+SHAPES = {0: (128, 128), 1: (128, 64), 2: (64, 64), 3: (128, 160), 4: (64, 128), 5: (256, 128),
+          6: (128, 256)}
 
 
 def stream_handle(stream: Optional[torch.cuda.Stream] = None) -> int:
@@ -45,12 +46,12 @@ def pick_shape(M: int, N: int) -> int:
 def make_desc(A: int, B: int, C: int, M: int, N: int, K: int, sAm: int, sAk: int, sBk: int, sBn: int,
               sCm: int, sCn: int, batch: int = 1, sAb: int = 0, sBb: int = 0, sCb: int = 0,
               alpha: float = 1.0, beta: float = 0.0, diag: float = 0.0, flags: int = 0, ksplit: int = 1,
-              skip: int = 0, scale_a: int = 0, scale_b: int = 0) -> L.GemmDesc:
+              skip: int = 0, scale_a: int = 0, scale_b: int = 0, sCk: int = 0) -> L.GemmDesc:
     d = L.GemmDesc()
     d.A, d.B, d.C = A, B, C
     d.M, d.N, d.K = M, N, K
     d.sAm, d.sAk, d.sBk, d.sBn, d.sCm, d.sCn = sAm, sAk, sBk, sBn, sCm, sCn
-    d.sAb, d.sBb, d.sCb = sAb, sBb, sCb
+    d.sAb, d.sBb, d.sCb, d.sCk = sAb, sBb, sCb, sCk
     d.batch = batch
     d.flags = flags | ((ksplit if ksplit > 1 else 0) << 8)
     d.alpha, d.beta, d.diag = alpha, beta, diag

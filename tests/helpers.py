@@ -1,0 +1,69 @@
+"""Dataset builders shared by the tests (the oracle is the checker here)."""
+import math
+
+import numpy as np
+
+from fasthigashi import synthetic as syn
+from oracle import ref_np as R
+
+
+def build_dataset(wl, bs_bin=None, bs_cell=None, do_rwr=None, n_i=None, dt=np.float32):
+    """Oracle pipeline: synthetic COO -> packed Sparse -> tiles -> dense -> partial RWR.
+
+    Returns (chrom_data list, raw dict with packed COO per chromosome, n_i list)."""
+    do_rwr = wl.do_rwr if do_rwr is None else do_rwr
+    tab = syn.generate(wl)
+    data, raw, nis = [], [], []
+    for c, _ in enumerate(wl.bins):
+        mats = syn.per_cell_csr(tab, c, wl.n_cells)
+        idx, val, shape, mapping = R.pack_chrom(mats, wl.off_diag)
+        sidx, sval, indptr = R.sort_indices(idx, val, shape)
+        n = shape[0]
+        bsb = n if bs_bin is None else min(bs_bin, n)
+        tiles = R.tile_plan(n, bsb, wl.off_diag)
+        N = shape[2]
+        bsc = N if bs_cell is None else bs_cell
+        chunks = [slice(k, min(k + bsc, N)) for k in range(0, N, bsc)]
+        X, ni_c = [], []
+        for tl in tiles:
+            parts = []
+            for ch in chunks:
+                coo, v, ps = R.tile_coo(sidx, sval, indptr, shape, tl, ch)
+                d = R.densify(coo, v, ps, transpose=True, dt=dt)
+                if do_rwr:
+                    x, ni = R.partial_rwr(d, tl.local_start, tl.local_start + tl.rows, False, True, False,
+                                          force_rwr_epochs=-1 if n_i is None else n_i[c])
+                    ni_c.append(ni)
+                else:
+                    x = d.transpose(1, 2, 0)
+                parts.append(np.ascontiguousarray(x))
+            X.append(np.concatenate(parts, axis=2))
+        nis.append(max(ni_c) if ni_c else 0)
+        data.append(R.ChromData("chr%d" % (c + 1), n, wl.resolution, tiles, X, bsb, chunks))
+        raw.append(dict(idx=idx, val=val, shape=shape, sidx=sidx, sval=sval, indptr=indptr))
+    return data, raw, nis
+
+
+def random_init(data, wl, rank, seed=0):
+    """Seeded init with the reference's structure (parafac2_intergrative.py:71-80,283-290):
+    A ~ 1 + 1e-2 randn, B = I + 1e-2 randn(s) broadcast over rows, (M, D) from the SVD of a
+    random cell x sum(s) matrix."""
+    rng = np.random.default_rng(seed)
+    sizes = [R.size_of(cd.num_bin, wl.dim1, wl.resolution, rank) for cd in data]
+    A = [(rng.standard_normal((cd.num_bin, s)) * 1e-2 + 1).astype(np.float32) for cd, s in zip(data, sizes)]
+    B = [(np.eye(s) + 1e-2 * rng.standard_normal(s)[None, :]).astype(np.float32) for s in sizes]
+    N = data[0].X[0].shape[2]
+    C = rng.standard_normal((N, sum(sizes))).astype(np.float32)
+    U, S, Vh = np.linalg.svd(C, full_matrices=False)
+    M = U[:, :rank].astype(np.float32)
+    SVh = (S[:rank, None] * Vh[:rank]).astype(np.float32)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    D = [SVh[:, offs[i]:offs[i + 1]].copy() for i in range(len(sizes))]
+    return R.Factors(A, B, D, M), sizes
+
+
+def canonical_corr(M1, M2):
+    """Singular values of Q1^T Q2 for orthonormal bases of the two column spaces."""
+    Q1, _ = np.linalg.qr(np.asarray(M1, dtype=np.float64))
+    Q2, _ = np.linalg.qr(np.asarray(M2, dtype=np.float64))
+    return np.linalg.svd(Q1.T @ Q2, compute_uv=False)

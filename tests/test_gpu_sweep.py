@@ -1,0 +1,68 @@
+"""GPU ALS sweep (plain FP32 and BF16) against the numpy oracle from the same seeded init."""
+import numpy as np
+import pytest
+import torch
+
+from fasthigashi import synthetic as syn
+from tests.helpers import build_dataset, canonical_corr, random_init
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_workload():
+    wl = syn.WORKLOADS["c1"]
+    return syn.scaled(wl, n_cells=160, bins=[120, 100, 90], rank=32, off_diag=30, band=30, name="c1_tiny")
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    wl = _small_workload()
+    data, raw, nis = build_dataset(wl, bs_bin=50)
+    F0, sizes = random_init(data, wl, wl.rank, seed=3)
+    return wl, data, F0, sizes
+
+
+def _engine(data, F0, sizes, rank, dtype):
+    from fasthigashi.engine import SweepEngine, TileSpec
+    tiles = [[TileSpec(t.start, t.rows, t.width, t.col_start, t.local_start, t.shift) for t in cd.tiles]
+             for cd in data]
+    eng = SweepEngine(tiles, [cd.num_bin for cd in data], sizes, data[0].X[0].shape[2], rank, dtype=dtype)
+    eng.set_X_tiles([cd.X for cd in data])
+    eng.set_factors(F0.A, F0.B, F0.D, F0.M)
+    return eng
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp32", 1e-4), ("bf16", 1e-3)])
+def test_sweep_matches_oracle(cuda, tiny, dtype, tol):
+    from oracle import ref_np as R
+    wl, data, F0, sizes = tiny
+    n_it = 12            # crosses the n_iter_parafac 1 -> 2 switch at iteration 10
+    ref = R.fit(data, F0, n_it, 1, 0.0, np.float32 if dtype == "fp32" else np.float64)
+    eng = _engine(data, F0, sizes, wl.rank, dtype)
+    for _ in range(n_it):
+        eng.sweep()
+    torch.cuda.synchronize()
+    h = eng.history()
+    rec_ref = np.array(ref.rec_errors)
+    rel = np.abs(h["rec_error"] - rec_ref) / rec_ref
+    assert rel.max() < tol, (rel, h["rec_error"], rec_ref)
+    xv_ref = np.array(ref.x_V)
+    assert np.max(np.abs(h["x_V"] - xv_ref) / np.abs(xv_ref)) < tol
+    xu_ref = np.array(ref.x_U)
+    assert np.max(np.abs(h["x_U"] - xu_ref) / np.abs(xu_ref)) < tol
+    A, B, D, M = eng.get_factors()
+    cc = canonical_corr(M, ref.factors.M)
+    assert cc.min() > 0.999, cc
+
+
+def test_sweep_fp64_oracle_fit(cuda, tiny):
+    """Plain FP32 GPU path vs the FP64 oracle: per-iteration fit within 1e-4."""
+    from oracle import ref_np as R
+    wl, data, F0, sizes = tiny
+    ref64 = R.fit(data, F0, 6, 1, 0.0, np.float64)
+    eng = _engine(data, F0, sizes, wl.rank, "fp32")
+    for _ in range(6):
+        eng.sweep()
+    h = eng.history()
+    rel = np.abs(h["rec_error"] - np.array(ref64.rec_errors)) / np.array(ref64.rec_errors)
+    assert rel.max() < 1e-4, rel
