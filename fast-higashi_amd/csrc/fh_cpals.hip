@@ -1,0 +1,223 @@
+// Inner CP-ALS on every chromosome's projected tensor at once (parafac_integrative.py:12-112).
+//
+// For T_c (n x s x R, stored (n*sp) x R) and factors A_c (n x s), B_c (s x s), D_c (R x s):
+//   mode A:  A = [sum_j W1[i,j,:] B[j,:]] (B^T B o D^T D + 1e-10 I)^{-1},   W1 = T x_3 D
+//   mode B:  B = [sum_i W1[i,j,:] A[i,:]] (A^T A o D^T D + 1e-10 I)^{-1}
+//   mode D:  D = [T^T (A (.) B)]          (A^T A o B^T B + 1e-10 I)^{-1}
+// The GEMMs (W1, Grams, T^T KR, the solves as products with the FP64 inverse) run as
+// grouped launches over all chromosomes; this file holds the table-driven element-wise
+// and reduction steps between them.  blockIdx.y selects the chromosome.
+#include "fh_common.h"
+
+extern "C" {
+typedef struct FhCpChrom {
+  float* A;
+  float* B;
+  float* D;
+  const float* T;
+  float* W1;
+  float* MF;
+  float* KR;
+  const float* MDpart;
+  double* G;        // 4 x s x sp: A^T A, B^T B, D^T D, system matrix
+  double* scal;     // [||T||^2, <M_D, D>, ||[[A,B,D]]||^2, spare]
+  double* state;    // [previous loss, have previous]
+  int* flag;        // inner loop stopped
+  double* core;     // output slot for the core norm
+  int64_t n, s, sp, R, Dld, mdsplit;
+} FhCpChrom;
+}
+
+namespace {
+constexpr int NT = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t += red[w];
+  return t;
+}
+
+// mode 0: H = GB o GD, 1: GA o GD, 2: GA o GB
+__global__ void cp_hadamard_kernel(const FhCpChrom* __restrict__ tab, int mode) {
+  const FhCpChrom q = tab[blockIdx.y];
+  const int64_t s = q.s, sp = q.sp, sz = s * sp;
+  const double* g1 = q.G + (mode == 0 ? 1 : 0) * sz;
+  const double* g2 = q.G + (mode == 2 ? 1 : 2) * sz;
+  double* h = q.G + 3 * sz;
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < s * s; e += (int64_t)gridDim.x * NT) {
+    int64_t i = e / s, j = e % s;
+    h[i * sp + j] = g1[i * sp + j] * g2[i * sp + j];
+  }
+}
+
+// mode 0: MF[i][r] = sum_j W1[i*sp+j][r] B[j][r];  1: MF[j][r] = sum_i W1[i*sp+j][r] A[i][r];
+// 2: MF[k][r] = sum_p MDpart[p][k][r]  (fixed-order reduction of the k-split partials)
+__global__ void cp_mttkrp_kernel(const FhCpChrom* __restrict__ tab, int mode) {
+  const FhCpChrom q = tab[blockIdx.y];
+  const int64_t n = q.n, s = q.s, sp = q.sp;
+  const int64_t rows = mode == 0 ? n : (mode == 1 ? s : q.R);
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < rows * s; e += (int64_t)gridDim.x * NT) {
+    int64_t a = e / s, r = e % s;
+    double acc = 0.;
+    if (mode == 0) {
+      for (int64_t j = 0; j < s; ++j) acc += (double)q.W1[(a * sp + j) * sp + r] * (double)q.B[j * sp + r];
+    } else if (mode == 1) {
+      for (int64_t i = 0; i < n; ++i) acc += (double)q.W1[(i * sp + a) * sp + r] * (double)q.A[i * sp + r];
+    } else {
+      for (int64_t p = 0; p < q.mdsplit; ++p) acc += (double)q.MDpart[(p * q.R + a) * sp + r];
+    }
+    q.MF[a * sp + r] = (float)acc;
+  }
+}
+
+// KR[r][i*sp + j] = A[i][r] B[j][r]  (zero for j >= s)
+__global__ void cp_krp_kernel(const FhCpChrom* __restrict__ tab) {
+  const FhCpChrom q = tab[blockIdx.y];
+  const int64_t cols = q.n * q.sp, tot = q.s * cols;
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t)gridDim.x * NT) {
+    int64_t r = e / cols, c = e % cols, i = c / q.sp, j = c % q.sp;
+    q.KR[e] = (j < q.s) ? (float)((double)q.A[i * q.sp + r] * (double)q.B[j * q.sp + r]) : 0.f;
+  }
+}
+
+// ||T_c||^2, partial sums: part[c * nblk + blockIdx.x]
+__global__ void cp_tnorm_kernel(const FhCpChrom* __restrict__ tab, double* part, int nblk) {
+  __shared__ double red[NT / 64];
+  const FhCpChrom q = tab[blockIdx.y];
+  const int64_t tot = q.n * q.sp * q.R;
+  double p = 0.;
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t)nblk * NT) {
+    double v = q.T[e];
+    p += v * v;
+  }
+  double t = block_sum(p, red);
+  if (threadIdx.x == 0) part[blockIdx.y * nblk + blockIdx.x] = t;
+}
+
+__global__ void cp_tnorm_finish_kernel(const FhCpChrom* __restrict__ tab, const double* part, int nblk) {
+  __shared__ double red[NT / 64];
+  const FhCpChrom q = tab[blockIdx.y];
+  double p = 0.;
+  for (int i = threadIdx.x; i < nblk; i += NT) p += part[blockIdx.y * nblk + i];
+  double t = block_sum(p, red);
+  if (threadIdx.x == 0) {
+    q.scal[0] = t;
+    q.state[0] = 0.;
+    q.state[1] = 0.;
+    *q.flag = 0;
+  }
+}
+
+// Stopping test of parafac_integrative.py:80-107 from closed forms:
+//   loss = ||T||^2 - 2 <M_D, D> + sum(A^T A o B^T B o D^T D);  stop when (prev - loss)/prev < 1e-5.
+__global__ void cp_check_kernel(const FhCpChrom* __restrict__ tab) {
+  __shared__ double red[NT / 64];
+  const FhCpChrom q = tab[blockIdx.y];
+  if (*q.flag) return;
+  const int64_t s = q.s, sp = q.sp, sz = s * sp;
+  double p = 0.;
+  for (int64_t e = threadIdx.x; e < q.R * s; e += NT) {
+    int64_t k = e / s, r = e % s;
+    p += (double)q.MF[k * sp + r] * (double)q.D[k * q.Dld + r];
+  }
+  const double inner = block_sum(p, red);
+  double h = 0.;
+  for (int64_t e = threadIdx.x; e < s * s; e += NT) {
+    int64_t i = e / s, j = e % s;
+    h += q.G[i * sp + j] * q.G[sz + i * sp + j] * q.G[2 * sz + i * sp + j];
+  }
+  const double xhat2 = block_sum(h, red);
+  if (threadIdx.x == 0) {
+    const double loss = q.scal[0] - 2. * inner + xhat2;
+    q.scal[1] = inner;
+    q.scal[2] = xhat2;
+    if (q.state[1] != 0.) {
+      const double wdiff = (q.state[0] - loss) / q.state[0];
+      if (wdiff < 1e-5) *q.flag = 1;
+    }
+    q.state[0] = loss;
+    q.state[1] = 1.;
+  }
+}
+
+// balance_norm (parafac_integrative.py:19-26): block (r, c) normalises column r of chromosome c
+__global__ void cp_balance_kernel(const FhCpChrom* __restrict__ tab, int use_flag) {
+  __shared__ double red[NT / 64];
+  const FhCpChrom q = tab[blockIdx.y];
+  const int64_t r = blockIdx.x;
+  if (r >= q.s) return;
+  if (use_flag && *q.flag) return;
+  double pa = 0., pb = 0., pd = 0.;
+  for (int64_t i = threadIdx.x; i < q.n; i += NT) { double v = q.A[i * q.sp + r]; pa += v * v; }
+  for (int64_t i = threadIdx.x; i < q.s; i += NT) { double v = q.B[i * q.sp + r]; pb += v * v; }
+  for (int64_t i = threadIdx.x; i < q.R; i += NT) { double v = q.D[i * q.Dld + r]; pd += v * v; }
+  const float na = (float)sqrt(block_sum(pa, red));
+  const float nb = (float)sqrt(block_sum(pb, red));
+  const float nd = (float)sqrt(block_sum(pd, red));
+  const float even = na * nb * nd + 1e-15f;
+  const float ia = 1.f / (na + 1e-15f), ib = 1.f / (nb + 1e-15f), id = even / (nd + 1e-15f);
+  for (int64_t i = threadIdx.x; i < q.n; i += NT) q.A[i * q.sp + r] *= ia;
+  for (int64_t i = threadIdx.x; i < q.s; i += NT) q.B[i * q.sp + r] *= ib;
+  for (int64_t i = threadIdx.x; i < q.R; i += NT) q.D[i * q.Dld + r] *= id;
+}
+
+// core norm sum(A^T A o B^T B o D^T D) -> *q.core
+__global__ void cp_core_kernel(const FhCpChrom* __restrict__ tab) {
+  __shared__ double red[NT / 64];
+  const FhCpChrom q = tab[blockIdx.y];
+  const int64_t s = q.s, sp = q.sp, sz = s * sp;
+  double h = 0.;
+  for (int64_t e = threadIdx.x; e < s * s; e += NT) {
+    int64_t i = e / s, j = e % s;
+    h += q.G[i * sp + j] * q.G[sz + i * sp + j] * q.G[2 * sz + i * sp + j];
+  }
+  const double t = block_sum(h, red);
+  if (threadIdx.x == 0) *q.core = t;
+}
+
+inline unsigned gx(int64_t n) {
+  int64_t g = (n + NT - 1) / NT;
+  return (unsigned)(g > 4096 ? 4096 : (g < 1 ? 1 : g));
+}
+}  // namespace
+
+extern "C" {
+
+// op: 0 hadamard(mode), 1 mttkrp(mode), 2 krp, 3 check, 4 balance(mode = use_flag), 5 core
+// max_elems: largest element count over the chromosomes for the op (grid sizing).
+int fh_cpals_op(const FhCpChrom* tab_dev, int K, int op, int mode, int64_t max_elems, hipStream_t st) {
+  if (K <= 0) return FH_OK;
+  switch (op) {
+    case 0: hipLaunchKernelGGL(cp_hadamard_kernel, dim3(gx(max_elems), K), dim3(NT), 0, st, tab_dev, mode); break;
+    case 1: hipLaunchKernelGGL(cp_mttkrp_kernel, dim3(gx(max_elems), K), dim3(NT), 0, st, tab_dev, mode); break;
+    case 2: hipLaunchKernelGGL(cp_krp_kernel, dim3(gx(max_elems), K), dim3(NT), 0, st, tab_dev); break;
+    case 3: hipLaunchKernelGGL(cp_check_kernel, dim3(1, K), dim3(NT), 0, st, tab_dev); break;
+    case 4:
+      FH_REQUIRE(max_elems >= 1 && max_elems < 65536, "cpals balance: bad column count");
+      hipLaunchKernelGGL(cp_balance_kernel, dim3((unsigned)max_elems, K), dim3(NT), 0, st, tab_dev, mode);
+      break;
+    case 5: hipLaunchKernelGGL(cp_core_kernel, dim3(1, K), dim3(NT), 0, st, tab_dev); break;
+    default: fh_set_error("cpals: unknown op %d", op); return FH_ERR_ARG;
+  }
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+// ||T_c||^2 into scal[0]; resets the stop flag and loss state.  part: K * nblk doubles.
+int fh_cpals_tnorm(const FhCpChrom* tab_dev, int K, double* part, int nblk, hipStream_t st) {
+  if (K <= 0) return FH_OK;
+  FH_REQUIRE(nblk >= 1 && nblk <= 4096, "cpals tnorm: nblk");
+  hipLaunchKernelGGL(cp_tnorm_kernel, dim3(nblk, K), dim3(NT), 0, st, tab_dev, part, nblk);
+  FH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(cp_tnorm_finish_kernel, dim3(1, K), dim3(NT), 0, st, tab_dev, part, nblk);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+}  // extern "C"

@@ -214,6 +214,15 @@ __global__ void batch_sum_kernel(const T* in, int64_t nb, int64_t stride, int64_
   }
 }
 
+// X[b][i][i] = v (set) or += v, i < n
+__global__ void diag_kernel(float* X, int64_t nb, int64_t stride, int64_t ld, int64_t n, float v, int add) {
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < nb * n; e += (int64_t)gridDim.x * NT) {
+    int64_t b = e / n, i = e % n;
+    float* p = X + b * stride + i * ld + i;
+    *p = add ? *p + v : v;
+  }
+}
+
 // CP-ALS inner-iteration test (parafac_integrative.py:80-108) from closed forms:
 //   loss = ||T||^2 - 2 <M_D, D> + ||[[A,B,D]]||^2 ; wdiff = (prev - loss) / prev; stop if wdiff < 1e-5
 // state: [0] = previous loss, [1] = have previous; flag: 1 once stopped.
@@ -348,6 +357,13 @@ int fh_batch_sum(const void* in, int dt, int64_t nb, int64_t stride, int64_t cou
   else
     hipLaunchKernelGGL((batch_sum_kernel<float>), dim3(grid_for(count)), dim3(NT), 0, st, (const float*)in, nb,
                        stride, count, (float*)out);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_diag(float* X, int64_t nb, int64_t stride, int64_t ld, int64_t n, float v, int add, hipStream_t st) {
+  if (nb * n <= 0) return FH_OK;
+  hipLaunchKernelGGL(diag_kernel, dim3(grid_for(nb * n)), dim3(NT), 0, st, X, nb, stride, ld, n, v, add);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }

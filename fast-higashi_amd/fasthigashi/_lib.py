@@ -56,6 +56,14 @@ class BandTile(Structure):
                 ("row_start", c_int32), ("rows", c_int32), ("shift", c_int32), ("width", c_int32)]
 
 
+class CpChrom(Structure):
+    _fields_ = [("A", c_void_p), ("B", c_void_p), ("D", c_void_p), ("T", c_void_p), ("W1", c_void_p),
+                ("MF", c_void_p), ("KR", c_void_p), ("MDpart", c_void_p), ("G", c_void_p), ("scal", c_void_p),
+                ("state", c_void_p), ("flag", c_void_p), ("core", c_void_p),
+                ("n", c_int64), ("s", c_int64), ("sp", c_int64), ("R", c_int64), ("Dld", c_int64),
+                ("mdsplit", c_int64)]
+
+
 class LossTile(Structure):
     _fields_ = [("X", c_void_p), ("P", c_void_p), ("Y", c_void_p), ("gY", c_void_p), ("gP", c_void_p),
                 ("w", c_int64), ("s", c_int64), ("n", c_int64), ("ldx", c_int64), ("ldp", c_int64),
@@ -95,7 +103,10 @@ _SIGS = {
                          c_void_p]),
     "fh_sweep_errors": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "fh_batch_sum": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "fh_diag": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_float, c_int, c_void_p]),
     "fh_cpals_check": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fh_cpals_op": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_void_p]),
+    "fh_cpals_tnorm": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "fh_csr_sort_workspace": (c_int64, [c_int64, c_int32, c_int64]),
     "fh_csr_sort": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_int64, c_void_p]),

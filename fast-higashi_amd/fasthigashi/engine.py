@@ -197,6 +197,7 @@ class SweepEngine:
                                 dtype=torch.int64, device=dev)
         self.iteration = 0
         self.n_iter_parafac = 1
+        self.timing = None         # set to [] to record per-phase HIP events
         self._build()
 
     # ------------------------------------------------------------------ views
@@ -238,7 +239,7 @@ class SweepEngine:
             self.B_view(c)[:, :cl.s] = torch.as_tensor(np.asarray(B[c], dtype=np.float32), device=self.device)
             self.D_view(c)[:, :cl.s] = torch.as_tensor(np.asarray(D[c], dtype=np.float32), device=self.device)
         self.M[:self.N] = torch.as_tensor(np.asarray(M, dtype=np.float32), device=self.device)
-        self._core_norms(self.core[self.core_cur])
+        self._core_norms(self.core_cur)
 
     def get_factors(self):
         A = [self.A_view(c)[:, :cl.s].cpu().numpy().copy() for c, cl in enumerate(self.chroms)]
@@ -370,83 +371,123 @@ class SweepEngine:
         self._build_cpals()
 
     def _build_cpals(self):
-        dev, R, Dld = self.device, self.R, self.Dld
+        dev, R, Dld, K = self.device, self.R, self.Dld, self.K
         f4 = lambda t, o=0: t.data_ptr() + 4 * o
-        self.cp = []
+        self.cp_part_blk = 64
+        self.cp_part = torch.zeros(K * self.cp_part_blk, dtype=torch.float64, device=dev)
+        tabs = [[], []]
+        w1, gA, gB, gD, uA, uB, uD, md, spd = [], [], [], [], [], [], [], [], []
         for c, cl in enumerate(self.chroms):
             n, s, sp = cl.n, cl.s, cl.sp
             skip = self.flags.data_ptr() + 4 * c
             Ap, Bp, Dp = f4(self.A_all, cl.aoff), f4(self.B_all, cl.boff), f4(self.D_all, cl.doff)
             Gr = self.Gr[c]
             gp = [Gr[i].data_ptr() for i in range(4)]
-            e = {}
-            # W1 = T (n sp x R) . D (R x s)
-            e["w1"] = ops.GemmGroup([make_desc(f4(self.Ttil, cl.ttoff), Dp, f4(self.W1[c]), n * sp, s, R,
-                                               R, 1, Dld, 1, sp, 1)], F32, shape_for_n(s), dev)
-            # Grams (FP64):  F^T F for F = A (n x s), B (s x s), D (R x s)
-            def gram(F, rows, ld, out):
-                return make_desc(F, F, out, s, s, rows, 1, ld, ld, 1, sp, 1)
-            e["gA"] = ops.Gemm64Group([gram(Ap, n, sp, gp[0])], F32, F32, F64, dev)
-            e["gB"] = ops.Gemm64Group([gram(Bp, s, sp, gp[1])], F32, F32, F64, dev)
-            e["gD"] = ops.Gemm64Group([gram(Dp, R, Dld, gp[2])], F32, F32, F64, dev)
+            for k in range(2):
+                q = L.CpChrom()
+                q.A, q.B, q.D, q.T = Ap, Bp, Dp, f4(self.Ttil, cl.ttoff)
+                q.W1, q.MF, q.KR, q.MDpart = f4(self.W1[c]), f4(self.MF[c]), f4(self.KRf[c]), f4(self.MDpart[c])
+                q.G = Gr.data_ptr()
+                q.scal = self.cp_scal.data_ptr() + 8 * 4 * c
+                q.state = self.cp_state.data_ptr() + 8 * 2 * c
+                q.flag = skip
+                q.core = self.core[k].data_ptr() + 8 * c
+                q.n, q.s, q.sp, q.R, q.Dld, q.mdsplit = n, s, sp, R, Dld, self.mdsplit[c]
+                tabs[k].append(q)
+            w1.append(make_desc(f4(self.Ttil, cl.ttoff), Dp, f4(self.W1[c]), n * sp, s, R, R, 1, Dld, 1, sp, 1))
+            gA.append(make_desc(Ap, Ap, gp[0], s, s, n, 1, sp, sp, 1, sp, 1))
+            gB.append(make_desc(Bp, Bp, gp[1], s, s, s, 1, sp, sp, 1, sp, 1))
+            gD.append(make_desc(Dp, Dp, gp[2], s, s, R, 1, Dld, Dld, 1, sp, 1))
+            MF, Hi = f4(self.MF[c]), self.Hinv[c].data_ptr()
+            uA.append(make_desc(MF, Hi, Ap, n, s, s, sp, 1, 1, sp, sp, 1, skip=skip))
+            uB.append(make_desc(MF, Hi, Bp, s, s, s, sp, 1, 1, sp, sp, 1, skip=skip))
+            uD.append(make_desc(MF, Hi, Dp, R, s, s, sp, 1, 1, sp, Dld, 1, skip=skip))
+            ks = self.mdsplit[c]
+            md.append(make_desc(f4(self.Ttil, cl.ttoff), f4(self.KRf[c]), f4(self.MDpart[c]), R, s, n * sp,
+                                1, R, 1, n * sp, sp, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=ks, sCk=R * sp))
             q = L.SpdSys()
             q.G, q.out, q.s, q.ldg, q.ldo, q.eps = gp[3], self.Hinv[c].data_ptr(), s, sp, sp, 1e-10
-            e["spd"] = ops._upload((L.SpdSys * 1)(q), dev)
-            MF = f4(self.MF[c])
-            Hi = self.Hinv[c].data_ptr()
-            # factor updates  F_new = MTTKRP (G + 1e-10 I)^{-1}   (skipped once the inner loop stopped)
-            e["uA"] = ops.Gemm64Group([make_desc(MF, Hi, Ap, n, s, s, sp, 1, 1, sp, sp, 1, skip=skip)],
-                                      F32, F64, F32, dev)
-            e["uB"] = ops.Gemm64Group([make_desc(MF, Hi, Bp, s, s, s, sp, 1, 1, sp, sp, 1, skip=skip)],
-                                      F32, F64, F32, dev)
-            e["uD"] = ops.Gemm64Group([make_desc(MF, Hi, Dp, R, s, s, sp, 1, 1, sp, Dld, 1, skip=skip)],
-                                      F32, F64, F32, dev)
-            # M_D = T^T KR  (R x s), K = n sp, split-k partials
-            ks = self.mdsplit[c]
-            e["md"] = ops.GemmGroup([make_desc(f4(self.Ttil, cl.ttoff), f4(self.KRf[c]), f4(self.MDpart[c]),
-                                               R, s, n * sp, 1, R, 1, n * sp, sp, 1,
-                                               flags=L.GEMM_SPLIT_PARTIAL if ks > 1 else 0, ksplit=ks,
-                                               sCk=R * sp)], F32, 6 if s > 160 else shape_for_n(s), dev)
-            self.cp.append(e)
+            spd.append(q)
+        self.cp_tab = [ops._upload((L.CpChrom * K)(*t), dev) for t in tabs]
+        self.spd_tab = ops._upload((L.SpdSys * K)(*spd), dev)
+        self.g_w1 = MultiGemm(w1, F32, lambda d: shape_for_n(d.N), dev)
+        self.g_gA = ops.Gemm64Group(gA, F32, F32, F64, dev)
+        self.g_gB = ops.Gemm64Group(gB, F32, F32, F64, dev)
+        self.g_gD = ops.Gemm64Group(gD, F32, F32, F64, dev)
+        self.g_gBD = ops.Gemm64Group(gB + gD, F32, F32, F64, dev)
+        self.g_gABD = ops.Gemm64Group(gA + gB + gD, F32, F32, F64, dev)
+        self.g_uA = ops.Gemm64Group(uA, F32, F64, F32, dev)
+        self.g_uB = ops.Gemm64Group(uB, F32, F64, F32, dev)
+        self.g_uD = ops.Gemm64Group(uD, F32, F64, F32, dev)
+        self.g_md = MultiGemm(md, F32, lambda d: 6 if d.N > 160 else shape_for_n(d.N), dev)
+        self.cp_max = dict(s2=max(c.s * c.s for c in self.chroms), ns=max(c.n * c.s for c in self.chroms),
+                           Rs=max(self.R * c.s for c in self.chroms), s=max(c.s for c in self.chroms),
+                           kr=max(c.s * c.n * c.sp for c in self.chroms))
 
     # ------------------------------------------------------------------ sweep
-    def _core_norms(self, out: torch.Tensor, stream=None):
-        st = ops.stream_handle(stream)
-        for c, cl in enumerate(self.chroms):
-            e = self.cp[c]
-            e["gA"].run(stream); e["gB"].run(stream); e["gD"].run(stream)
-            G = self.Gr[c]
-            L.call("fh_sum3", G[0].data_ptr(), G[1].data_ptr(), G[2].data_ptr(), cl.s, cl.sp,
-                   out.data_ptr() + 8 * c, st)
+    def _core_norms(self, which: int, stream=None):
+        """Core norms sum(A^T A o B^T B o D^T D) of every chromosome into self.core[which]."""
+        self.g_gABD.run(stream)
+        L.call("fh_cpals_op", self.cp_tab[which].data_ptr(), self.K, 5, 0, 1, ops.stream_handle(stream))
+
+    def _mark(self, name, stream):
+        if self.timing is not None:
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(stream if stream is not None else torch.cuda.current_stream())
+            self.timing.append((name, ev))
+
+    def phase_times(self):
+        """Milliseconds per phase of the timed sweeps (requires timing enabled)."""
+        out = {}
+        if not self.timing:
+            return out
+        torch.cuda.synchronize()
+        for (n0, e0), (n1, e1) in zip(self.timing[:-1], self.timing[1:]):
+            if n0 == "end":
+                continue
+            out[n0] = out.get(n0, 0.0) + e0.elapsed_time(e1)
+        return out
 
     def sweep(self, n_iter_parafac: Optional[int] = None, stream=None):
         """One ALS iteration; the errors land in self.hist[self.iteration] (device)."""
         st = ops.stream_handle(stream)
+        mk = lambda name: self._mark(name, stream)
         it = self.iteration
         if n_iter_parafac is None:
             if it % 10 == 0 and it > 0 and self.n_iter_parafac < 10:       # :636-637
                 self.n_iter_parafac += 1
             n_iter_parafac = self.n_iter_parafac
         # pass 1
+        mk("ct")
         self.g_ct.run(stream)
+        mk("mttkrp_y")
         self.g_y.run(stream)
+        mk("scale_temp")
         L.call("fh_scale_rows", self.rs_tab.data_ptr(), self.rs_n, self.rs_max, st)
         self.g_temp.run(stream)
+        mk("gram64")
         self.g_gram.run(stream)
+        mk("polar_ns")
         self.pb.solve(stream)
+        mk("polar_apply")
         self.g_p.run(stream)
         L.call("fh_segment_sum", self.pb.nuc.data_ptr(), self.seg.data_ptr(), self.K, self.xU.data_ptr(), 0, st)
         if self.dtype == "bf16":
             L.call("fh_copy2d", self.P.data_ptr(), F32, self.P.numel(), 1, self.P16.data_ptr(), BF16,
                    self.P.numel(), 1, self.P.numel(), 1.0, st)
         # pass 2
+        mk("that")
         self.g_that.run(stream)
+        mk("krp")
         for c, cl in enumerate(self.chroms):
             L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,
                    cl.sp, self.Lk.data_ptr() + self.Lk.element_size() * cl.loff, self.xdt, st)
+        mk("z")
         self.g_z.run(stream)
+        mk("svd_term")
         self.g_svd.run(stream)
         # meta-embedding polar
+        mk("meta_polar")
         self.g_gm.run(stream)
         L.call("fh_batch_sum", self.GMpart.data_ptr(), F64, self.meta_ksplit, self.R * self.R, self.R * self.R,
                self.Gm.data_ptr(), st)
@@ -456,77 +497,49 @@ class SweepEngine:
             L.call("fh_copy2d", self.M.data_ptr(), F32, self.R, 1, self.M16.data_ptr(), BF16, self.R,
                    self.Npad, self.R, 1.0, st)
         # pass 3
+        mk("ttilde")
         self.g_ttil.run(stream)
         # errors with the previous core norm (:667-668)
         prev = self.core[self.core_cur]
         L.call("fh_sweep_errors", self.tnorm.data_ptr(), prev.data_ptr(), self.xU.data_ptr(),
                self.pbm.nuc.data_ptr(), self.K, self.hist.data_ptr() + 8 * self.hist_w * it, st)
         # CP-ALS on every T_c, then the new core norm (:674-706)
+        mk("cpals")
         self._cpals(n_iter_parafac, stream)
         self.core_cur ^= 1
-        self._core_norms(self.core[self.core_cur], stream)
+        self._core_norms(self.core_cur, stream)
+        mk("end")
         self.iteration += 1
 
     def _cpals(self, n_iter: int, stream=None):
+        """parafac() on every chromosome (parafac_integrative.py:28-112), n_iter inner iterations."""
         st = ops.stream_handle(stream)
-        ops.fill_(self.flags, 0, stream)
-        ops.fill_(self.cp_state, 0.0, stream)
-        for c, cl in enumerate(self.chroms):
-            T = self.T_view(c)
-            L.call("fh_dot2d", T.data_ptr(), F32, self.R, T.data_ptr(), F32, self.R, cl.n * cl.sp, self.R,
-                   self.cp_scal.data_ptr() + 8 * (4 * c), st)
-        for c, cl in enumerate(self.chroms):
-            Ap, Bp, Dp = self.A_view(c), self.B_view(c), self.D_view(c)
-            L.call("fh_balance_norm", Ap.data_ptr(), cl.sp, cl.n, Bp.data_ptr(), cl.sp, Dp.data_ptr(), self.Dld,
-                   self.R, cl.s, None, st)                                          # :65
+        tab, K, mx = self.cp_tab[0].data_ptr(), self.K, self.cp_max
+        L.call("fh_cpals_tnorm", tab, K, self.cp_part.data_ptr(), self.cp_part_blk, st)
+        L.call("fh_cpals_op", tab, K, 4, 0, mx["s"], st)                       # balance_norm (:65)
         for _ in range(n_iter):
-            for c, cl in enumerate(self.chroms):
-                self._cpals_iter(c, st, stream)
+            self.g_w1.run(stream)                                              # W1 = T x_3 D
+            self.g_gBD.run(stream)
+            self._solve_mode(0, mx["ns"], self.g_uA, st, stream)
+            self.g_gA.run(stream)
+            self._solve_mode(1, mx["s2"], self.g_uB, st, stream)
+            self.g_gB.run(stream)
+            L.call("fh_cpals_op", tab, K, 0, 2, mx["s2"], st)                  # A^T A o B^T B
+            L.call("fh_spd_inverse", self.spd_tab.data_ptr(), K, mx["s"], self.spd_status.data_ptr(), st)
+            L.call("fh_cpals_op", tab, K, 2, 0, mx["kr"], st)                  # A (.) B
+            self.g_md.run(stream)                                              # T^T (A (.) B), k-split
+            L.call("fh_cpals_op", tab, K, 1, 2, mx["Rs"], st)                  # reduce the partials
+            self.g_uD.run(stream)
+            self.g_gD.run(stream)
+            L.call("fh_cpals_op", tab, K, 3, 0, 1, st)                         # stopping test (:80-107)
+            L.call("fh_cpals_op", tab, K, 4, 1, mx["s"], st)                   # balance unless stopped
 
-    def _cpals_iter(self, c: int, st, stream):
-        cl = self.chroms[c]
-        e = self.cp[c]
-        n, s, sp, R = cl.n, cl.s, cl.sp, self.R
-        G = self.Gr[c]
-        g = [G[i].data_ptr() for i in range(4)]
-        Ap, Bp, Dp = self.A_view(c).data_ptr(), self.B_view(c).data_ptr(), self.D_view(c).data_ptr()
-        flag = self.flags.data_ptr() + 4 * c
-        MF = self.MF[c].data_ptr()
-        e["w1"].run(stream)                                     # W1 = T x_3 D
-        e["gB"].run(stream); e["gD"].run(stream)
-        # mode A
-        L.call("fh_hadamard", g[1], g[2], None, s, sp, g[3], st)
-        L.call("fh_spd_inverse", e["spd"].data_ptr(), 1, s, self.spd_status.data_ptr(), st)
-        L.call("fh_mttkrp_reduce", self.W1[c].data_ptr(), sp, Bp, sp, n, s, sp, 0, MF, sp, st)
-        e["uA"].run(stream)
-        # mode B
-        e["gA"].run(stream)
-        L.call("fh_hadamard", g[0], g[2], None, s, sp, g[3], st)
-        L.call("fh_spd_inverse", e["spd"].data_ptr(), 1, s, self.spd_status.data_ptr(), st)
-        L.call("fh_mttkrp_reduce", self.W1[c].data_ptr(), sp, Ap, sp, n, s, sp, 1, MF, sp, st)
-        e["uB"].run(stream)
-        # mode D
-        e["gB"].run(stream)
-        L.call("fh_hadamard", g[0], g[1], None, s, sp, g[3], st)
-        L.call("fh_spd_inverse", e["spd"].data_ptr(), 1, s, self.spd_status.data_ptr(), st)
-        L.call("fh_krp", Ap, sp, Bp, sp, n, s, sp, self.KRf[c].data_ptr(), F32, st)
-        e["md"].run(stream)
-        ks = self.mdsplit[c]
-        if ks > 1:
-            L.call("fh_batch_sum", self.MDpart[c].data_ptr(), F32, ks, R * sp, R * sp, MF, st)
-            mdp = MF
-        else:
-            mdp = self.MDpart[c].data_ptr()
-        if ks <= 1:
-            L.call("fh_copy2d", mdp, F32, sp, 1, MF, F32, sp, R, sp, 1.0, st)
-        e["uD"].run(stream)
-        # stopping test from closed forms (:80-107), then balance unless stopped (:108)
-        e["gD"].run(stream)
-        sc = self.cp_scal.data_ptr() + 8 * (4 * c)
-        L.call("fh_dot2d", MF, F32, sp, Dp, F32, self.Dld, R, s, sc + 8, st)
-        L.call("fh_sum3", g[0], g[1], g[2], s, sp, sc + 16, st)
-        L.call("fh_cpals_check", sc, sc + 8, sc + 16, self.cp_state.data_ptr() + 16 * c, flag, st)
-        L.call("fh_balance_norm", Ap, sp, n, Bp, sp, Dp, self.Dld, R, s, flag, st)
+    def _solve_mode(self, mode: int, elems: int, upd, st, stream):
+        tab, K, mx = self.cp_tab[0].data_ptr(), self.K, self.cp_max
+        L.call("fh_cpals_op", tab, K, 0, mode, mx["s2"], st)                   # Gram Hadamard
+        L.call("fh_spd_inverse", self.spd_tab.data_ptr(), K, mx["s"], self.spd_status.data_ptr(), st)
+        L.call("fh_cpals_op", tab, K, 1, mode, elems, st)                      # MTTKRP from W1
+        upd.run(stream)
 
     # ------------------------------------------------------------------ results
     def history(self, n: Optional[int] = None):

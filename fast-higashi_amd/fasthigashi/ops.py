@@ -133,8 +133,9 @@ class Gemm64Group:
 
 
 def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alpha=1.0, beta=0.0,
-         shape=None, stream=None) -> torch.Tensor:
-    """C = alpha A @ B + beta C for 2-D (or 3-D batched) strided tensors on the device.
+         shape=None, stream=None, diag=0.0) -> torch.Tensor:
+    """C = alpha A @ B + beta C (+ diag on the diagonal) for 2-D (or 3-D batched) strided
+    tensors on the device.
 
     Inputs fp32 / bf16 (both the same) run on the MFMA kernel; fp64 inputs on the
     FP64 kernel.  One of the two strides of each operand must be 1."""
@@ -150,7 +151,7 @@ def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alp
                         device=A.device)
     d = make_desc(A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, A.stride(1), A.stride(2),
                   B.stride(1), B.stride(2), C.stride(1), C.stride(2), nb, A.stride(0), B.stride(0),
-                  C.stride(0), alpha, beta)
+                  C.stride(0), alpha, beta, diag)
     if A.dtype == torch.float64 or B.dtype == torch.float64 or C.dtype == torch.float64:
         Gemm64Group([d], DT_CODE[A.dtype], DT_CODE[B.dtype], DT_CODE[C.dtype]).run(stream)
     else:

@@ -1,0 +1,335 @@
+"""scHi-C contacts -> packed COO -> dense band tiles in HBM (with partial RWR).
+
+Three stages, mirroring the reference's data path:
+
+* `pack_chrom` — host restatement of FastHigashi.pack_training_data_one_process
+  (FastHigashi_Wrapper.py:221-366): band filter, valid-bin remap, coverage
+  normalisation, log1p, clip at mean + 15 std.  Vectorised over cells; the output
+  (int32 [row, col, cell] indices and float32 values, cell-major order) is identical
+  to the per-cell loop of the reference because every coverage sum is a sum of
+  integer counts (exact in float32 below 2^24) and the value array is produced in
+  the reference's order before log1p / mean / std / clip.
+* `synth_chrom_device` — the same packing for large synthetic workloads computed on
+  the device (used to build benchmark inputs quickly; values agree with `pack_chrom`
+  up to float32 rounding of the mean / std reduction).
+* `TileBuilder` — per (tile, cell chunk): band scatter of the COO into a dense
+  cell-major tile (absent entries = 1e-8, sparse_for_schic.py:279-320), partial RWR
+  (partial_rwr.py:45-175) on the matrix cores, transpose into the sweep's
+  cell-innermost X layout.  Also produces the init-time pooled features and bin
+  coverage (parafac2_intergrative.py:110-158).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import ops
+from .ops import rup
+
+
+# ---------------------------------------------------------------------------
+# host packing (exact)
+# ---------------------------------------------------------------------------
+
+def symmetric_entries(cell, a, b, count):
+    """Entries of csr(a, b) + csr(a, b)^T per cell (Fast_process.py:135-136), cell-major,
+    row-major and column-ascending within a cell (the order of .tocoo() on that sum)."""
+    cell = np.asarray(cell, dtype=np.int64)
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    v = np.asarray(count, dtype=np.float32)
+    off = a != b
+    rows = np.concatenate([a, b[off]])
+    cols = np.concatenate([b, a[off]])
+    cells = np.concatenate([cell, cell[off]])
+    vals = np.concatenate([np.where(off, v, v + v), v[off]]).astype(np.float32)
+    order = np.lexsort((cols, rows, cells))
+    return cells[order], rows[order], cols[order], vals[order]
+
+
+def pack_chrom(cell, a, b, count, n_bins: int, n_cells: int, off_diag: int):
+    """Packed COO of one chromosome (FastHigashi_Wrapper.py:253-366, merge_fac=1, fac_size=1,
+    no blacklist, no batch_id).  Input: upper-triangle contact table (a <= b) of every cell.
+
+    Returns (indices int32 [3, nnz] = (row, col, cell), values float32, shape (n_valid, n_valid,
+    n_cells), mapping int64 (n_bins,) with -1 for dropped bins)."""
+    cells, rows, cols, vals = symmetric_entries(cell, a, b, count)
+    keep = np.abs(cols - rows) <= off_diag                              # :265-269
+    cells, rows, cols, vals = cells[keep], rows[keep], cols[keep], vals[keep]
+    rowsum = np.bincount(rows, weights=vals.astype(np.float64), minlength=n_bins)   # :278-281
+    valid = rowsum > 0                          # preprocessing.py:478 (c > min(0, 0.01 l))
+    mapping = np.cumsum(valid) - 1
+    mapping[~valid] = -1
+    n_valid = int(valid.sum())
+    # coverage normalisation per cell (preprocessing.py:137-142): sums of integer counts
+    sums = np.bincount(cells, weights=vals.astype(np.float64), minlength=n_cells).astype(np.float32)
+    factor = np.float32(n_bins) / (sums + np.float32(1e-15))
+    vals = vals * factor[cells]
+    r, c = mapping[rows], mapping[cols]
+    ok = (r != -1) & (c != -1)                                          # :314-318
+    idx = np.empty((3, int(ok.sum())), dtype=np.int32)
+    idx[0] = r[ok]
+    idx[1] = c[ok]
+    idx[2] = cells[ok]
+    vals = vals[ok]
+    vals = np.log1p(vals)                                               # :358-364
+    mean_, std_ = np.mean(vals), np.std(vals)
+    vals = np.clip(vals, a_min=None, a_max=mean_ + 15 * std_)
+    return idx, vals, (n_valid, n_valid, n_cells), mapping
+
+
+# ---------------------------------------------------------------------------
+# device synthetic packing (benchmark inputs)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class DeviceCOO:
+    """Packed COO of one chromosome on the device, rows CSR-sorted (indptr)."""
+    idx: torch.Tensor        # int32 (3, nnz): row, col, cell
+    val: torch.Tensor        # float32 (nnz,)
+    indptr: torch.Tensor     # int64 (n + 1,) on the device
+    indptr_host: np.ndarray
+    n: int
+    n_cells: int
+
+
+def csr_sort_device(idx: torch.Tensor, val: torch.Tensor, n_rows: int, chunk: int = 4096):
+    """Stable sort by row on the device (sparse_for_schic.py:133-182 semantics)."""
+    nnz = idx.shape[1]
+    nnz_chunks = max(1, -(-nnz // chunk))
+    while nnz_chunks * n_rows > 64 * 1024 * 1024 and chunk < (1 << 24):
+        chunk *= 2
+        nnz_chunks = max(1, -(-nnz // chunk))
+    ws = L.lib().fh_csr_sort_workspace(nnz, n_rows, chunk)
+    wsb = torch.empty(ws, dtype=torch.uint8, device=idx.device)
+    oi = torch.empty_like(idx)
+    ov = torch.empty_like(val)
+    ptr = torch.empty(n_rows + 1, dtype=torch.int64, device=idx.device)
+    L.call("fh_csr_sort", idx.data_ptr(), val.data_ptr(), nnz, n_rows, chunk, oi.data_ptr(), ov.data_ptr(),
+           ptr.data_ptr(), wsb.data_ptr(), ws, ops.stream_handle())
+    return oi, ov, ptr
+
+
+def synth_device(wl, device="cuda", cell_batch: int = 1024) -> List[DeviceCOO]:
+    """Generate a workload's contacts on the device and pack every chromosome.
+
+    Same model as synthetic.generate (compartments, lognormal totals, multinomial over
+    band pairs with weight (1+d)^-1.1 (1 + 0.3 u_a u_b)), drawn with a device RNG seeded
+    from the workload seed; then the packing of `pack_chrom` on the device."""
+    from . import synthetic as syn
+    dev = torch.device(device)
+    rng = np.random.default_rng(wl.seed)
+    pairs = [syn.pair_table(n, wl.band) for n in wl.bins]
+    pair_off = np.concatenate([[0], np.cumsum([len(p[0]) for p in pairs])])
+    comp = [[syn.compartments(rng, n) for n in wl.bins] for _ in range(wl.n_types)]
+    base = [np.power(1.0 + (b - a), -1.1) for a, b in pairs]
+    cdfs = []
+    for t in range(wl.n_types):
+        w = np.concatenate([bw * (1.0 + 0.3 * comp[t][c][pairs[c][0]] * comp[t][c][pairs[c][1]])
+                            for c, bw in enumerate(base)])
+        cdf = np.cumsum(w)
+        cdfs.append(torch.tensor(cdf / cdf[-1], dtype=torch.float64, device=dev))
+    cell_type = np.arange(wl.n_cells) % wl.n_types
+    rng.shuffle(cell_type)
+    totals = np.maximum(np.rint(rng.lognormal(math.log(wl.mu), wl.sigma, size=wl.n_cells)).astype(np.int64), 1)
+    P = int(pair_off[-1])
+    a_all = torch.tensor(np.concatenate([p[0] for p in pairs]), dtype=torch.int64, device=dev)
+    b_all = torch.tensor(np.concatenate([p[1] for p in pairs]), dtype=torch.int64, device=dev)
+    chrom_of_pair = torch.tensor(np.repeat(np.arange(len(wl.bins)), np.diff(pair_off)), dtype=torch.int64,
+                                 device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(wl.seed) * 7919 + 17)
+    key_cnt = []
+    for c0 in range(0, wl.n_cells, cell_batch):
+        c1 = min(c0 + cell_batch, wl.n_cells)
+        tot = torch.tensor(totals[c0:c1], device=dev)
+        ct = torch.tensor(cell_type[c0:c1], device=dev)
+        cells = torch.repeat_interleave(torch.arange(c0, c1, device=dev), tot)
+        types = torch.repeat_interleave(ct, tot)
+        u = torch.rand(cells.shape[0], generator=gen, device=dev, dtype=torch.float64)
+        pidx = torch.empty_like(cells)
+        for t in range(wl.n_types):
+            m = types == t
+            if m.any():
+                pidx[m] = torch.searchsorted(cdfs[t], u[m], right=True).clamp_(max=P - 1)
+        k, cnt = torch.unique(cells * P + pidx, return_counts=True)
+        key_cnt.append((k, cnt))
+        del u, cells, types, pidx
+    key = torch.cat([k for k, _ in key_cnt])
+    cnt = torch.cat([c for _, c in key_cnt]).to(torch.float32)
+    del key_cnt
+    cell = key // P
+    pair = key % P
+    chrom = chrom_of_pair[pair]
+    res = []
+    for c, n in enumerate(wl.bins):
+        m = chrom == c
+        ce, pa, cn = cell[m], pair[m], cnt[m]
+        a = a_all[pa]
+        b = b_all[pa]
+        off = a != b
+        rows = torch.cat([a, b[off]])
+        cols = torch.cat([b, a[off]])
+        cells = torch.cat([ce, ce[off]])
+        vals = torch.cat([torch.where(off, cn, cn + cn), cn[off]])
+        keep = (cols - rows).abs() <= wl.off_diag
+        rows, cols, cells, vals = rows[keep], cols[keep], cells[keep], vals[keep]
+        valid = torch.bincount(rows, minlength=n) > 0
+        mapping = torch.cumsum(valid.to(torch.int64), 0) - 1
+        mapping[~valid] = -1
+        n_valid = int(valid.sum().item())
+        sums = torch.zeros(wl.n_cells, dtype=torch.float64, device=dev).index_add_(0, cells, vals.double())
+        factor = torch.tensor(np.float32(n), device=dev) / (sums.float() + 1e-15)
+        vals = vals * factor[cells]
+        r, cc = mapping[rows], mapping[cols]
+        ok = (r >= 0) & (cc >= 0)
+        idx = torch.stack([r[ok], cc[ok], cells[ok]]).to(torch.int32).contiguous()
+        vals = torch.log1p(vals[ok])
+        mean_ = vals.double().mean()
+        std_ = vals.double().std(unbiased=False)
+        vals = torch.clamp(vals, max=float(mean_ + 15 * std_)).contiguous()
+        del rows, cols, cells
+        oi, ov, ptr = csr_sort_device(idx, vals, n_valid)
+        del idx, vals
+        res.append(DeviceCOO(oi, ov, ptr, ptr.cpu().numpy(), n_valid, wl.n_cells))
+    return res
+
+
+def coo_to_device(idx: np.ndarray, val: np.ndarray, shape, device="cuda", presorted=False) -> DeviceCOO:
+    """Upload a packed COO and CSR-sort it on the device (no-op order change if presorted)."""
+    i = torch.as_tensor(np.ascontiguousarray(idx, dtype=np.int32), device=device)
+    v = torch.as_tensor(np.ascontiguousarray(val, dtype=np.float32), device=device)
+    oi, ov, ptr = csr_sort_device(i, v, int(shape[0]))
+    return DeviceCOO(oi, ov, ptr, ptr.cpu().numpy(), int(shape[0]), int(shape[2]))
+
+
+# ---------------------------------------------------------------------------
+# dense tiles + partial RWR on the device
+# ---------------------------------------------------------------------------
+
+class TileBuilder:
+    """Dense (cells, rows, w) tiles of one chromosome and their partial RWR."""
+
+    def __init__(self, coo: DeviceCOO, device="cuda", budget_bytes: int = 8 << 30):
+        self.coo = coo
+        self.dev = torch.device(device)
+        self.budget = budget_bytes
+        self.tile_of_row = torch.zeros(max(coo.n, 1), dtype=torch.int32, device=self.dev)
+
+    def chunk_cells(self, rows: int, width: int) -> int:
+        wp, rp = rup(width, 8), rup(rows, 8)
+        per_cell = 4 * (2 * rows * wp + 3 * rows * rp)
+        return max(1, min(self.coo.n_cells, self.budget // per_cell))
+
+    def dense(self, tile, cells: slice, fill: float = 1e-8) -> torch.Tensor:
+        """(nc, rows, wp) fp32 tile: 1e-8 where no contact, padding columns zero."""
+        rows, w = tile.rows, tile.width
+        wp = rup(w, 8)
+        nc = cells.stop - cells.start
+        Xc = torch.empty(nc, rows, wp, dtype=torch.float32, device=self.dev)
+        ops.fill_(Xc, fill)
+        if wp > w:
+            Xc[:, :, w:] = 0.0
+        bt = L.BandTile()
+        bt.offset, bt.s_row, bt.s_col, bt.s_cell = 0, wp, 1, rows * wp
+        bt.row_start, bt.rows, bt.shift, bt.width = tile.start, rows, tile.shift, w
+        tab = ops._upload((L.BandTile * 1)(bt), self.dev)
+        lo = int(self.coo.indptr_host[tile.start])
+        hi = int(self.coo.indptr_host[min(tile.start + rows, self.coo.n)])
+        nnz = hi - lo
+        if nnz > 0:
+            tot = self.coo.idx.shape[1]
+            # the kernel reads idx rows at stride nnz: hand it a compact copy of the tile's range
+            sub = self.coo.idx[:, lo:hi].contiguous()
+            L.call("fh_band_scatter", sub.data_ptr(), self.coo.val.data_ptr() + 4 * lo, nnz,
+                   self.tile_of_row.data_ptr(), tab.data_ptr(), cells.start, cells.stop, Xc.data_ptr(), L.F32,
+                   None, ops.stream_handle())
+        return Xc
+
+    def rwr(self, Xc: torch.Tensor, tile, epochs: int, do_col: bool = False,
+            bin_cov: Optional[torch.Tensor] = None, cells: Optional[slice] = None):
+        """Partial RWR of a dense tile (partial_rwr.py:83-138).  epochs < 0: auto-stop
+        (max over cells of ||Q_k - Q_{k+1}||_F < 0.01, at most 60 epochs).
+
+        Returns (out (nc, rows, wp) fp32, epoch_count)."""
+        nc, rows, wp = Xc.shape
+        w = tile.width
+        rp = rup(rows, 8)
+        ls = tile.local_start
+        S2 = torch.zeros(nc, rows, rp, dtype=torch.float32, device=self.dev)
+        ops.gemm(Xc, Xc.transpose(1, 2), S2[:, :, :rows])                     # :85
+        Pm = torch.zeros_like(S2)
+        L.call("fh_rwr_prep", S2.data_ptr(), rp, rows * rp, Xc.data_ptr(), wp, rows * wp, ls, rows,
+               Pm.data_ptr(), rp, rows * rp, nc, ops.stream_handle())        # :86-111
+        Q = S2                                                                  # reuse: Q_0 = I
+        ops.fill_(Q, 0.0)
+        L.call("fh_diag", Q.data_ptr(), nc, rows * rp, rp, rows, 1.0, 0, ops.stream_handle())
+        Qn = torch.zeros_like(Q)
+        auto = epochs < 0
+        n_ep = 60 if auto else epochs
+        count = 0
+        delta = torch.empty(nc, dtype=torch.float32, device=self.dev)
+        dmax = torch.empty(1, dtype=torch.float32, device=self.dev)
+        for _ in range(n_ep):                                                  # :115-126
+            ops.gemm(Q[:, :, :rows], Pm[:, :, :rows], Qn[:, :, :rows], alpha=0.5, diag=0.5)
+            if auto:
+                L.call("fh_rwr_delta", Q.data_ptr(), Qn.data_ptr(), rp, rows * rp, rows, nc, delta.data_ptr(),
+                       dmax.data_ptr(), ops.stream_handle())
+                Q, Qn = Qn, Q
+                if float(dmax.item()) < 0.01:
+                    break
+            else:
+                Q, Qn = Qn, Q
+            count += 1
+        A = Xc
+        if do_col:                                                             # :131-135
+            L.call("fh_rwr_symnorm", Q.data_ptr(), Qn.data_ptr(), rp, rows * rp, rows, nc, ops.stream_handle())
+            Q, Qn = Qn, Q
+            n = bin_cov.shape[1]
+            cov = bin_cov[cells.start:cells.stop, tile.col_start: tile.col_start + w]
+            L.call("fh_div_cols", A.data_ptr(), wp, rows * wp, rows, w, cov.data_ptr(), n, nc,
+                   ops.stream_handle())
+        out = torch.zeros(nc, rows, wp, dtype=torch.float32, device=self.dev)
+        ops.gemm(Q[:, :, :rows], A[:, :, :w], out[:, :, :w])                  # :138
+        return out, count
+
+    def store(self, out: torch.Tensor, tile, X_tile: torch.Tensor, cell0: int, dtype_code: int):
+        """Transpose (nc, rows, wp) into the (rows*w, Npad) cell-innermost tile at column cell0."""
+        nc, rows, wp = out.shape
+        w = tile.width
+        key = (rows, w, wp)
+        if not hasattr(self, "_pm") or self._pm[0] != key:
+            p = np.arange(rows * wp)
+            r, col = p // wp, p % wp
+            pm = np.where(col < w, r * w + col, -1).astype(np.int64)
+            self._pm = (key, torch.as_tensor(pm, device=self.dev))
+        Npad = X_tile.shape[-1]
+        dst = X_tile.reshape(rows * w, Npad)
+        L.call("fh_transpose", out.data_ptr(), rows * wp, nc, rows * wp,
+               dst.data_ptr() + dst.element_size() * cell0, dtype_code, Npad, self._pm[1].data_ptr(),
+               ops.stream_handle())
+
+
+def build_X(engine, coos: Sequence[DeviceCOO], n_i: Sequence[int], do_rwr: bool = True, do_col: bool = False,
+            bin_cov: Optional[Sequence[torch.Tensor]] = None, budget_bytes: int = 8 << 30):
+    """Fill engine.X with the post-RWR tiles of every chromosome (fixed epochs n_i[c])."""
+    code = engine.xdt
+    for c, (cl, coo) in enumerate(zip(engine.chroms, coos)):
+        tb = TileBuilder(coo, engine.device, budget_bytes)
+        for t, tl in enumerate(cl.tiles):
+            nc_max = tb.chunk_cells(tl.rows, tl.width)
+            for k0 in range(0, engine.N, nc_max):
+                cells = slice(k0, min(k0 + nc_max, engine.N))
+                Xc = tb.dense(tl, cells)
+                if do_rwr:
+                    out, _ = tb.rwr(Xc, tl, int(n_i[c]), do_col, bin_cov[c] if bin_cov is not None else None, cells)
+                else:
+                    out = Xc
+                tb.store(out, tl, engine.X_tile(c, t), k0, code)
+                del Xc, out
+    engine.finish_X()
