@@ -1,0 +1,264 @@
+"""Throughput of the PARAFAC2 ALS sweep (Fast_Higashi_core.fit loop body) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload c2] [--cells N]
+
+A step is one ALS sweep over the whole dataset resident in HBM.  The workload (c2 by
+default) is 10,000 synthetic cells x 22 chromosomes (500 -> 95 bins at 500 kb, band
+200), rank 256, partial RWR on.  Both the plain FP32 path and the fused BF16 path run on
+the same packed, RWR-imputed data; `value` is the BF16 path's cell*chrom sweeps/s and the
+FP32 figure is reported beside it.  Prints one JSON line on stdout; progress on stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fast-higashi_amd"))
+
+METRIC = "cell·chrom ALS sweeps/s at 10k cells r256: fused BF16 vs plain FP32, equal fit"
+PEAK = {"hbm": 8.0e12, "bf16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def phase_work(eng, phase, polar_iters):
+    """Algorithmic work of one launch of a sweep phase: (bound, amount, unit-per-second name).
+
+    mttkrp_y:  X_c C_c              2 W_c N s_c flop; streams X once (W_c N bytes)
+    that:      P_i^T X_i            2 W_c N s_c flop; streams X once
+    z:         T_hat^T L^T          2 n_c s_c N s_c flop; streams T_hat once
+    ttilde:    T_hat M              2 n_c s_c N R flop; streams T_hat once
+    polar_ns:  Newton-Schulz        sum over matrices of iterations * 3 * 2 s^3 flop (FP64)
+    """
+    N, R = eng.N, eng.R
+    xe = 2 if eng.dtype == "bf16" else 4
+    if phase in ("mttkrp_y", "that"):
+        fl = sum(2.0 * c.W * N * c.s for c in eng.chroms)
+        by = sum(float(c.W) * eng.Npad * xe for c in eng.chroms)
+    elif phase == "z":
+        fl = sum(2.0 * c.n * c.s * N * c.s for c in eng.chroms)
+        by = sum(float(c.n * c.sp) * eng.Npad * xe for c in eng.chroms)
+    elif phase == "ttilde":
+        fl = sum(2.0 * c.n * c.s * N * R for c in eng.chroms)
+        by = sum(float(c.n * c.sp) * eng.Npad * xe for c in eng.chroms)
+    elif phase == "polar_ns":
+        fl = 0.0
+        off = 0
+        for c in eng.chroms:
+            it = polar_iters[off: off + c.n].astype("float64")
+            fl += float((it + 1).sum()) * 2.0 * c.s ** 3 + float(it.sum()) * 4.0 * c.s ** 3
+            off += c.n
+        return "mfma", fl, "fp64"
+    else:
+        return None, 0.0, None
+    peak_key = "bf16" if eng.dtype == "bf16" else "fp32"
+    ai = fl / by
+    ridge = PEAK[peak_key] / PEAK["hbm"]
+    if ai < ridge:
+        return "hbm", by, None
+    return "mfma", fl, peak_key
+
+
+def build(args):
+    import numpy as np
+    import torch
+    from fasthigashi import synthetic as syn
+    from fasthigashi import tiling
+    from fasthigashi.engine import SweepEngine
+    from fasthigashi.packing import TileBuilder, build_X, synth_device
+    from fasthigashi import ops, _lib as L
+
+    wl = syn.WORKLOADS[args.workload]
+    if args.cells:
+        wl = syn.scaled(wl, n_cells=args.cells, name=wl.name)
+    t0 = time.time()
+    coos = synth_device(wl)
+    torch.cuda.synchronize()
+    nnz = sum(c.idx.shape[1] for c in coos)
+    log("generated + packed %d cells, %d chromosomes, nnz %d in %.1fs" % (wl.n_cells, len(coos), nnz,
+                                                                          time.time() - t0))
+    tiles = [tiling.tile_plan(c.n, tiling.bs_bin_for(c.n, wl.resolution), wl.off_diag) for c in coos]
+    sizes = [tiling.size_of(c.n, wl.dim1, wl.resolution, wl.rank) for c in coos]
+    n_bins = [c.n for c in coos]
+    # RWR epochs per chromosome: the auto-stop count of the init pass (parafac2_intergrative.py:124-140,265)
+    t0 = time.time()
+    n_i = []
+    if wl.do_rwr:
+        for c, coo in enumerate(coos):
+            tb = TileBuilder(coo)
+            m = 0
+            for tl in tiles[c]:
+                nc = tb.chunk_cells(tl.rows, tl.width)
+                for k0 in range(0, wl.n_cells, nc):
+                    cells = slice(k0, min(k0 + nc, wl.n_cells))
+                    _, cnt = tb.rwr(tb.dense(tl, cells), tl, -1)
+                    m = max(m, cnt)
+            n_i.append(m)
+        torch.cuda.synchronize()
+    log("RWR auto-stop pass %.1fs, epochs %s" % (time.time() - t0, n_i))
+    A, B, D, M = tiling.structured_random_init(n_bins, sizes, wl.n_cells, wl.rank, seed=wl.seed)
+    engines = {}
+    for dt in args.paths:
+        t0 = time.time()
+        eng = SweepEngine(tiles, n_bins, sizes, wl.n_cells, wl.rank, dtype=dt)
+        if "fp32" in engines:
+            src = engines["fp32"].X
+            L.call("fh_copy2d", src.data_ptr(), L.F32, src.numel(), 1, eng.X.data_ptr(), eng.xdt, src.numel(),
+                   1, src.numel(), 1.0, ops.stream_handle())
+            eng.finish_X()
+        else:
+            build_X(eng, coos, n_i, do_rwr=wl.do_rwr)
+        eng.set_factors(A, B, D, M)
+        torch.cuda.synchronize()
+        engines[dt] = eng
+        log("%s engine ready in %.1fs (HBM allocated %.1f GB)" % (dt, time.time() - t0,
+                                                                  torch.cuda.memory_allocated() / 1e9))
+    return wl, coos, tiles, sizes, n_i, (A, B, D, M), engines
+
+
+def time_path(eng, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        eng.sweep()
+    torch.cuda.synchronize()
+    eng.timing = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.sweep()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    phases = {k: v / steps for k, v in eng.phase_times().items()}
+    eng.timing = None
+    return el, phases
+
+
+def cpu_baseline(args, wl, coos, tiles, sizes, n_i, init, eng32):
+    """The oracle (numpy FP32 restatement, reference CPU tiling: one tile per chromosome
+    with all cells) timed on a bounded sample: one chromosome, the first S cells.  As
+    written, the reference recomputes the partial RWR of every tile on each of its two
+    fetches per sweep (parafac2_intergrative.py:349-365,500-516); that cost is added."""
+    import numpy as np
+    import torch
+    from fasthigashi import tiling
+    from fasthigashi.packing import TileBuilder
+    sys.path.insert(0, ROOT)
+    from oracle import ref_np as R
+
+    c = len(coos) // 2 if args.cpu_chrom < 0 else args.cpu_chrom
+    S = min(args.cpu_cells, wl.n_cells)
+    n = coos[c].n
+    tl = tiling.tile_plan(n, n, wl.off_diag)[0]        # CPU tiling: whole chromosome
+    tb = TileBuilder(coos[c])
+    dense = tb.dense(tl, slice(0, S))[:, :, :tl.width].cpu().numpy()   # (S, n, n) fp32, 1e-8 floor
+    threads = torch.get_num_threads()
+    t0 = time.perf_counter()
+    x, _ = R.partial_rwr(dense.copy(), tl.local_start, tl.local_start + tl.rows, False, wl.do_rwr, False,
+                         force_rwr_epochs=n_i[c] if wl.do_rwr else 0)
+    t_rwr = time.perf_counter() - t0
+    A, B, D, M = init
+    otile = R.Tile(tl.start, tl.rows, tl.width, tl.col_start, tl.local_start, tl.shift)
+    data = [R.ChromData("chr%d" % (c + 1), n, wl.resolution, [otile], [np.ascontiguousarray(x)], n,
+                        [slice(0, S)])]
+    F = R.Factors([A[c]], [B[c]], [D[c]], np.linalg.qr(M[:S])[0].astype(np.float32))
+    t0 = time.perf_counter()
+    R.fit(data, F, 1, 1, 0.0, np.float32)
+    t_sweep = time.perf_counter() - t0
+    total = t_sweep + 2 * t_rwr
+    return {"value": S / total, "unit": "cell·chrom sweeps/s", "cores": threads, "kind": "port",
+            "sample": "chromosome %d (%d bins) x first %d cells, one sweep, reference CPU tiling; %.2fs sweep "
+                      "+ 2 x %.2fs partial RWR per sweep (recomputed per fetch, as written)"
+                      % (c + 1, n, S, t_sweep, t_rwr)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cells", type=int, default=0, help="override the cell count (0: the workload's)")
+    ap.add_argument("--paths", default="fp32,bf16")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cells", type=int, default=1000)
+    ap.add_argument("--cpu-chrom", type=int, default=-2, help="-2: the median-size chromosome")
+    args = ap.parse_args()
+    args.paths = [p for p in args.paths.split(",") if p]
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sweep would shard by "
+              "(chromosome, row-tile) with one all-reduce of SVD_term per sweep); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    torch.cuda.init()
+    wl, coos, tiles, sizes, n_i, init, engines = build(args)
+    res = {}
+    for dt in args.paths:
+        el, phases = time_path(engines[dt], args.steps, args.warmup)
+        thr = wl.n_cells * len(coos) * args.steps / el
+        hist = engines[dt].history()
+        res[dt] = dict(elapsed=el, thr=thr, phases=phases, fit=float(hist["rec_error"][-1]))
+        log("%s: %.1f ms/sweep, %.4g cell*chrom sweeps/s, fit %.6f" % (dt, 1000 * el / args.steps, thr,
+                                                                      res[dt]["fit"]))
+        log("%s phases (ms/sweep): %s" % (dt, ", ".join("%s %.2f" % kv for kv in
+                                                        sorted(phases.items(), key=lambda kv: -kv[1]))))
+    head = "bf16" if "bf16" in res else args.paths[0]
+    eng = engines[head]
+    phases = res[head]["phases"]
+    polar_iters = eng.pb.iters.cpu().numpy()
+    dominant = max((p for p in phases if phase_work(eng, p, polar_iters)[0] is not None), key=lambda p: phases[p])
+    bound, amount, peak_key = phase_work(eng, dominant, polar_iters)
+    sec = phases[dominant] / 1000.0
+    if bound == "hbm":
+        roof = {"kernel": dominant, "bound": "hbm", "achieved": amount / sec / 1e9, "peak": PEAK["hbm"] / 1e9,
+                "unit": "GB/s", "frac": amount / sec / PEAK["hbm"], "traffic": None}
+    else:
+        roof = {"kernel": dominant, "bound": "mfma", "achieved": amount / sec / 1e12,
+                "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s", "frac": amount / sec / PEAK[peak_key],
+                "traffic": None, "precision": peak_key}
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        cpu = cpu_baseline(args, wl, coos, tiles, sizes, n_i, init, engines.get("fp32"))
+        log("cpu baseline %.1fs: %s" % (time.time() - t0, cpu))
+    out = {
+        "metric": METRIC,
+        "value": res[head]["thr"],
+        "unit": "cell·chrom sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000 * res[head]["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head,
+        "data": "synthetic (seeded scHi-C generator, packed and RWR-imputed on the device; seeded "
+                "reference-structured init)",
+        "config": {"workload": "%s: %d cells x %d chromosomes (%d -> %d bins at %d kb), band %d, rank %d, "
+                               "partial RWR %s, dim1 %.1f" % (wl.name, wl.n_cells, len(coos), max(wl.bins),
+                                                               min(wl.bins), wl.resolution // 1000, wl.band,
+                                                               wl.rank, "on" if wl.do_rwr else "off", wl.dim1),
+                   "cells": wl.n_cells, "chromosomes": len(coos), "rank": wl.rank,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "paths": {dt: {"cell_chrom_sweeps_per_s": r["thr"], "ms_per_sweep": 1000 * r["elapsed"] / args.steps,
+                       "fit_last": r["fit"], "phase_ms": {k: round(v, 3) for k, v in r["phases"].items()}}
+                  for dt, r in res.items()},
+    }
+    if "fp32" in res and "bf16" in res:
+        out["bf16_over_fp32"] = res["bf16"]["thr"] / res["fp32"]["thr"]
+        out["fit_rel_diff_bf16_fp32"] = abs(res["bf16"]["fit"] - res["fp32"]["fit"]) / res["fp32"]["fit"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

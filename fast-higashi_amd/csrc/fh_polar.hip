@@ -307,6 +307,203 @@ __global__ __launch_bounds__(NT) void spd_inverse_kernel(const FhSpdSys* __restr
   }
 }
 
+
+// Full s x s product  out = alpha * L * R (+ diag I) by one workgroup of GM x GN waves,
+// each owning TM x TN MFMA f64 16x16x4 tiles (s <= 16 TM GM = 16 TN GN), so every operand
+// element is loaded once per product.  Accumulates sum((out - I)^2) into *res if set.
+template <int GM, int GN, int TM, int TN>
+__device__ __attribute__((noinline)) void full_gemm_f64(const double* __restrict__ L, const double* __restrict__ R,
+                                              double* __restrict__ out, int64_t s, int64_t ld, double alpha,
+                                              double diag, double* res, double* smem) {
+  constexpr int TILE = 16 * TM * GM, BKD = 16, LR = BKD + 1, NTH = 64 * GM * GN;
+  static_assert(16 * TN * GN == TILE, "square tile");
+  double* As = smem;
+  double* Bs = smem + TILE * LR;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % GM, wn = wave / GM;
+  doublex4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = doublex4{0., 0., 0., 0.};
+  constexpr int PER = TILE * BKD / NTH;
+  static_assert(PER * NTH == TILE * BKD, "panel must split evenly over the threads");
+  double ra[PER], rb[PER];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int e = tid + p * NTH;
+      const int r = e >> 4, k = e & 15;
+      ra[p] = (r < s && k0 + k < s) ? L[r * ld + k0 + k] : 0.;
+      const int n = e % TILE, kk = e / TILE;
+      rb[p] = (n < s && k0 + kk < s) ? R[(k0 + kk) * ld + n] : 0.;
+    }
+  };
+  load(0);
+  for (int64_t k0 = 0; k0 < s; k0 += BKD) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int e = tid + p * NTH;
+      As[(e >> 4) * LR + (e & 15)] = ra[p];
+      Bs[(e % TILE) * LR + e / TILE] = rb[p];
+    }
+    __syncthreads();
+    if (k0 + BKD < s) load(k0 + BKD);     // next panel in flight during the MFMAs
+#pragma unroll 1     // unrolling hoists every LDS fragment load and spills the accumulators
+    for (int kk = 0; kk < BKD; kk += 4) {
+      double a[TM], b[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) a[i] = As[(wm * 16 * TM + i * 16 + (lane & 15)) * LR + kk + (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) b[j] = Bs[(wn * 16 * TN + j * 16 + (lane & 15)) * LR + kk + (lane >> 4)];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  double part = 0.;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int64_t row = wm * 16 * TM + i * 16 + (lane >> 4) + 4 * r;   // f64 MFMA output row mapping
+        int64_t col = wn * 16 * TN + j * 16 + (lane & 15);
+        if (row < s && col < s) {
+          double v = alpha * acc[i][j][r] + (row == col ? diag : 0.);
+          out[row * ld + col] = v;
+          double dv = v - (row == col ? 1. : 0.);
+          part += dv * dv;
+        }
+      }
+  if (res) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+    if (lane == 0) atomicAdd(res, part);
+  }
+}
+
+// wave grids of the fused solve per size class
+template <int C> struct NsCfg;
+template <> struct NsCfg<2> { static constexpr int GM = 2, GN = 2, TM = 2, TN = 2; };   //  64 x  64,  4 waves
+template <> struct NsCfg<4> { static constexpr int GM = 2, GN = 4, TM = 4, TN = 2; };   // 128 x 128,  8 waves
+template <> struct NsCfg<5> { static constexpr int GM = 2, GN = 5, TM = 5, TN = 2; };   // 160 x 160, 10 waves
+template <> struct NsCfg<0> { static constexpr int GM = 2, GN = 2, TM = 2, TN = 2; };   // tiled 64 x 64
+
+// size class of a matrix for the fused solve: full-matrix products for s <= 160
+__device__ __forceinline__ int ns_class(int64_t s) { return s <= 64 ? 2 : (s <= 128 ? 4 : (s <= 160 ? 5 : 0)); }
+
+// One workgroup per matrix runs the whole Newton-Schulz solve: scale, iterate until
+// ||T - I||_F^2 < tol (or max_iter), then Z <- Z / sqrt(c) into Z0 and the nuclear
+// norm.  Y/Z/T live in global memory (L2 / infinity cache); every phase is a sequence
+// of 64x64 output tiles computed by the four waves together.
+template <int WT>
+__global__ __launch_bounds__(64 * NsCfg<WT>::GM * NsCfg<WT>::GN) void ns_fused_kernel(
+    const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin, int max_iter,
+    double tol, double* __restrict__ nuc, int* __restrict__ iters) {
+  using Cfg = NsCfg<WT>;
+  constexpr int NTH = 64 * Cfg::GM * Cfg::GN;
+  __shared__ double smem[2 * 16 * Cfg::TM * Cfg::GM * 17];
+  __shared__ double red[16];
+  __shared__ double res_sh;
+  const int64_t gm = blockIdx.x;
+  int lo = 0, hi = n_groups - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (mat_begin[mid] <= gm) lo = mid; else hi = mid - 1;
+  }
+  const FhNsGroup g = groups[lo];
+  const int64_t b = gm - mat_begin[lo];
+  const int64_t s = g.s, ld = g.ld;
+  if (ns_class(s) != WT) return;          // another launch of this solve handles this size
+  double* const Y0 = g.Y0 + b * g.stride;
+  double* const Y1 = g.Y1 + b * g.stride;
+  double* const Z0 = g.Z0 + b * g.stride;
+  double* const Z1 = g.Z1 + b * g.stride;
+#define YB(i) ((i) ? Y1 : Y0)
+#define ZB(i) ((i) ? Z1 : Z0)
+  double* T = g.T + b * g.stride;
+  const int tid = threadIdx.x;
+  // scale: c = ||G||_F (G is in Y0)
+  double part = 0.;
+  for (int64_t e = tid; e < s * s; e += NTH) {
+    double v = Y0[(e / s) * ld + e % s];
+    part += v * v;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+  if ((tid & 63) == 0) red[tid >> 6] = part;
+  __syncthreads();
+  double c = 0.;
+  for (int w = 0; w < NTH / 64; ++w) c += red[w];
+  c = sqrt(c);
+  if (!(c > 0.)) c = 1.;
+  const double inv = 1. / c;
+  for (int64_t e = tid; e < s * s; e += NTH) {
+    int64_t r = e / s, col = e % s;
+    Y0[r * ld + col] *= inv;
+    Z0[r * ld + col] = (r == col) ? 1. : 0.;
+  }
+  __threadfence_block();
+  __syncthreads();
+  const int tm = (int)((s + 63) / 64);
+  int cur = 0, it = 0;
+  for (; it < max_iter; ++it) {
+    if (tid == 0) res_sh = 0.;
+    __syncthreads();
+    // T = 1.5 I - 0.5 Z Y
+    if constexpr (WT > 0) {
+      full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(ZB(cur), YB(cur), T, s, ld, -0.5, 1.5, &res_sh, smem);
+    } else {
+      for (int t = 0; t < tm * tm; ++t)
+        tile_gemm_f64(ZB(cur), YB(cur), T, s, ld, (t / tm) * 64, (t % tm) * 64, -0.5, 1.5, &res_sh, smem);
+    }
+    __threadfence_block();
+    __syncthreads();
+    const double r = res_sh;
+    if (r < tol) break;
+    // Y' = Y T, Z' = T Z
+    if constexpr (WT > 0) {
+      full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(YB(cur), T, YB(cur ^ 1), s, ld, 1.0, 0.0, nullptr, smem);
+      full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(T, ZB(cur), ZB(cur ^ 1), s, ld, 1.0, 0.0, nullptr, smem);
+    } else {
+      for (int t = 0; t < tm * tm; ++t) {
+        tile_gemm_f64(YB(cur), T, YB(cur ^ 1), s, ld, (t / tm) * 64, (t % tm) * 64, 1.0, 0.0, nullptr, smem);
+        tile_gemm_f64(T, ZB(cur), ZB(cur ^ 1), s, ld, (t / tm) * 64, (t % tm) * 64, 1.0, 0.0, nullptr, smem);
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    cur ^= 1;
+  }
+  // finalize into Z0: G^{-1/2} = Z / sqrt(c); nuclear norm = sqrt(c) trace(Y)
+  const double rs = sqrt(c), irs = 1. / rs;
+  part = 0.;
+  for (int64_t i = tid; i < s; i += NTH) part += YB(cur)[i * ld + i];
+  for (int64_t e = tid; e < s * s; e += NTH) {
+    int64_t rr = e / s, col = e % s;
+    Z0[rr * ld + col] = ZB(cur)[rr * ld + col] * irs;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = part;
+  __syncthreads();
+  if (tid == 0) {
+    double tr = 0.;
+    for (int w = 0; w < NTH / 64; ++w) tr += red[w];
+    nuc[gm] = rs * tr;
+    g.scale[b] = c;
+    if (iters) iters[gm] = it;
+  }
+#undef YB
+#undef ZB
+}
+
 }  // namespace
 
 extern "C" int64_t fh_ns_prepare(FhNsGroup* groups, int n) {
@@ -319,6 +516,27 @@ extern "C" int64_t fh_ns_prepare(FhNsGroup* groups, int n) {
     total += (int64_t)g.tiles * g.count;
   }
   return total;
+}
+
+// Fused solve: one launch, one workgroup per matrix; the result G^{-1/2} is left in Z0 and the
+// iteration count of every matrix in iters (nullable).
+extern "C" int fh_ns_fused(const FhNsGroup* groups_dev, int n_groups, const int64_t* mat_begin_dev,
+                           int64_t total_mats, int max_iter, double tol, double* nuc_dev, int* iters_dev,
+                           hipStream_t stream) {
+  FH_REQUIRE(max_iter >= 1 && max_iter <= 200, "ns_fused: max_iter out of range");
+  if (total_mats == 0) return FH_OK;
+  FH_REQUIRE(total_mats < (1ll << 31), "ns_fused: too many matrices");
+  const dim3 grid((unsigned)total_mats);
+  hipLaunchKernelGGL(ns_fused_kernel<5>, grid, dim3(640), 0, stream, groups_dev, n_groups, mat_begin_dev, max_iter,
+                     tol, nuc_dev, iters_dev);
+  hipLaunchKernelGGL(ns_fused_kernel<4>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev, max_iter,
+                     tol, nuc_dev, iters_dev);
+  hipLaunchKernelGGL(ns_fused_kernel<2>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev, max_iter,
+                     tol, nuc_dev, iters_dev);
+  hipLaunchKernelGGL(ns_fused_kernel<0>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev, max_iter,
+                     tol, nuc_dev, iters_dev);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
 }
 
 // Full Newton-Schulz run: G (in Y0 of each group) -> Z0/Z1 holds G^{-1/2}, nuc = trace(G^{1/2}).

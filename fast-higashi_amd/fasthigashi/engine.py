@@ -317,8 +317,8 @@ class SweepEngine:
                                     1, cl.sp, cl.sp, 1, G.stride(1), 1, tl.rows, tl.width * cl.sp,
                                     tl.width * cl.sp, G.stride(0)))
         self.g_gram = ops.Gemm64Group(d5, F32, F32, F64, dev)
-        # (7) P_i = temp_i G_i^{-1/2}
-        self.pb.result = self.zbuf
+        # (7) P_i = temp_i G_i^{-1/2}  (the fused solve leaves G^{-1/2} in buffer 0)
+        self.pb.result = 0
         d7 = []
         for c, cl in enumerate(self.chroms):
             Z = self.pb.Zinv(c)
@@ -468,7 +468,7 @@ class SweepEngine:
         mk("gram64")
         self.g_gram.run(stream)
         mk("polar_ns")
-        self.pb.solve(stream)
+        self.pb.solve_fused(stream)
         mk("polar_apply")
         self.g_p.run(stream)
         L.call("fh_segment_sum", self.pb.nuc.data_ptr(), self.seg.data_ptr(), self.K, self.xU.data_ptr(), 0, st)

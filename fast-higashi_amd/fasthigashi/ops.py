@@ -259,6 +259,14 @@ class PolarBatch:
         _, m = self.offs[g]
         return self.nuc[m: m + self.counts[g]]
 
+    def solve_fused(self, stream=None):
+        """One launch, one workgroup per matrix; G^{-1/2} ends in buffer 0."""
+        if not hasattr(self, "iters"):
+            self.iters = torch.zeros(self.nmat, dtype=torch.int32, device=self.buf.device)
+        L.call("fh_ns_fused", self.table.data_ptr(), len(self.groups), self.mat_begin.data_ptr(), self.nmat,
+               self.n_iter, self.tol, self.nuc.data_ptr(), self.iters.data_ptr(), stream_handle(stream))
+        self.result = 0
+
     def solve(self, stream=None):
         r = ctypes.c_int(0)
         L.call("fh_ns_run", self.table.data_ptr(), len(self.groups), self.mat_begin.data_ptr(),
@@ -281,7 +289,7 @@ def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24):
     d = make_desc(Yc.data_ptr(), Yc.data_ptr(), G.data_ptr(), s, s, m, 1, s, s, 1, G.stride(1), 1, b,
                   m * s, m * s, G.stride(0))
     Gemm64Group([d], DT_CODE[Yc.dtype], DT_CODE[Yc.dtype], L.F64).run()
-    pb.solve()
+    pb.solve_fused()
     Z = pb.Zinv(0)
     P = torch.empty((b, m, s), dtype=torch.float32, device=Y.device)
     d2 = make_desc(Yc.data_ptr(), Z.data_ptr(), P.data_ptr(), m, s, s, s, 1, 1, Z.stride(1), s, 1, b, m * s,

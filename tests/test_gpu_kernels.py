@@ -209,3 +209,23 @@ def test_csr_sort_matches_oracle(cuda):
         assert np.array_equal(oi.cpu().numpy(), ridx)
         assert np.array_equal(ov.cpu().numpy(), rval)
         assert np.array_equal(ptr.cpu().numpy(), rptr)
+
+
+def test_polar_fused_matches_multilaunch(cuda):
+    """The one-launch Newton-Schulz solve and the per-iteration launches agree."""
+    from fasthigashi import ops, _lib as L
+    rng = np.random.default_rng(4)
+    sizes, counts = [30, 70, 150], [3, 2, 2]
+    pbs = [ops.PolarBatch(sizes, counts, 48, 1e-24, cuda) for _ in range(2)]
+    for g, (s, c) in enumerate(zip(sizes, counts)):
+        X = rng.standard_normal((c, s + 20, s)) * np.exp(rng.uniform(-6, 0, size=(c, 1, s)))
+        G = torch.from_numpy(np.einsum("bmi,bmj->bij", X, X)).to(cuda)
+        for pb in pbs:
+            pb.G(g)[:, :, :s] = G
+    pbs[0].solve()
+    pbs[1].solve_fused()
+    for g, s in enumerate(sizes):
+        a, b = pbs[0].Zinv(g).cpu().numpy(), pbs[1].Zinv(g).cpu().numpy()
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max()
+    assert np.allclose(pbs[0].nuc.cpu().numpy(), pbs[1].nuc.cpu().numpy(), rtol=1e-12)
+    assert (pbs[1].iters.cpu().numpy() < 48).all()
