@@ -182,9 +182,10 @@ int64_t fh_csr_sort_workspace(int64_t nnz, int32_t n_rows, int64_t chunk) {
 
 // Stable sort of COO (3 x nnz int32 indices, row first; nnz float values) by row.
 // Outputs sorted indices/values and indptr (n_rows + 1, int64).
+// perm_out (nullable, nnz int64): source position of every sorted entry.
 int fh_csr_sort(const int32_t* idx, const float* val, int64_t nnz, int32_t n_rows, int64_t chunk,
-                int32_t* out_idx, float* out_val, int64_t* indptr, void* workspace, int64_t ws_bytes,
-                hipStream_t st) {
+                int32_t* out_idx, float* out_val, int64_t* indptr, int64_t* perm_out, void* workspace,
+                int64_t ws_bytes, hipStream_t st) {
   FH_REQUIRE(n_rows > 0 && n_rows <= 16384, "csr_sort: n_rows=%d out of range", n_rows);
   FH_REQUIRE(chunk >= 64 && chunk % 64 == 0, "csr_sort: chunk must be a multiple of 64");
   FH_REQUIRE(ws_bytes >= fh_csr_sort_workspace(nnz, n_rows, chunk), "csr_sort: workspace too small");
@@ -192,7 +193,7 @@ int fh_csr_sort(const int32_t* idx, const float* val, int64_t nnz, int32_t n_row
   char* ws = (char*)workspace;
   int32_t* hist = (int32_t*)ws;
   int64_t* total = (int64_t*)(ws + ((n_chunks > 0 ? n_chunks : 1) * n_rows * 4 + 7) / 8 * 8);
-  int64_t* perm = total + (n_rows + 1);
+  int64_t* perm = perm_out ? perm_out : total + (n_rows + 1);
   if (n_chunks > 0) {
     hipLaunchKernelGGL(chunk_hist_kernel, dim3((unsigned)n_chunks), dim3(NT), n_rows * 4, st, idx, nnz, chunk,
                        n_rows, hist);

@@ -135,6 +135,23 @@ __global__ void colsum_add_kernel(const float* __restrict__ X, int64_t ldX, int6
   }
 }
 
+// F.avg_pool2d(x, 3, stride 1, padding 1, ceil_mode=True) with count_include_pad, then
+// clamp(min=1e-8) (partial_rwr.py:77-81): out[c][i][k] = max(sum_{3x3} x / 9, 1e-8)
+__global__ void avgpool3_kernel(const float* __restrict__ X, int64_t ldX, int64_t strX, int64_t rows, int64_t w,
+                                int64_t ncell, float* __restrict__ out, int64_t ldO, int64_t strO) {
+  const int64_t tot = ncell * rows * w;
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t)gridDim.x * NT) {
+    int64_t c = e / (rows * w), q = e % (rows * w), i = q / w, k = q % w;
+    float s = 0.f;
+    for (int di = -1; di <= 1; ++di)
+      for (int dk = -1; dk <= 1; ++dk) {
+        int64_t ii = i + di, kk = k + dk;
+        if (ii >= 0 && ii < rows && kk >= 0 && kk < w) s += X[c * strX + ii * ldX + kk];
+      }
+    out[c * strO + i * ldO + k] = fmaxf(s / 9.f, 1e-8f);
+  }
+}
+
 inline unsigned gsz(int64_t n) {
   int64_t g = (n + NT - 1) / NT;
   return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
@@ -185,6 +202,15 @@ int fh_pool(const float* X, int64_t ldX, int64_t strX, int64_t rows, int64_t w, 
   FH_REQUIRE(ll >= 1, "pool: ll");
   hipLaunchKernelGGL(pool_kernel, dim3(gsz(ncell * (rows / ll) * (w / ll))), dim3(NT), 0, st, X, ldX, strX, rows, w,
                      ll, ncell, F, ldf);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+int fh_avgpool3(const float* X, int64_t ldX, int64_t strX, int64_t rows, int64_t w, int64_t ncell, float* out,
+                int64_t ldO, int64_t strO, hipStream_t st) {
+  if (ncell * rows * w <= 0) return FH_OK;
+  hipLaunchKernelGGL(avgpool3_kernel, dim3(gsz(ncell * rows * w)), dim3(NT), 0, st, X, ldX, strX, rows, w, ncell, out,
+                     ldO, strO);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }

@@ -1,0 +1,400 @@
+"""FastHigashi: the user-facing wrapper (drop-in for FastHigashi_Wrapper.py).
+
+Same constructor, methods and attributes as the reference.  Reads the per-chromosome
+cell matrices the reference's preprocessing writes (temp_dir/raw/<chrom>_sparse_adj.npy,
+an object array of scipy sparse matrices, and data_dir/label_info.pickle), packs them on
+the host (packing.pack_matrix_list), CSR-sorts and tiles them on the device
+(sparse_for_schic.Chrom_Dataset) and fits the decomposition with Fast_Higashi_core.
+
+Differences that do not change results: the packed-input cache is a .npz per
+resolution (indices/values/shape per chromosome) instead of a pickle stream, and the
+free device memory comes from hipMemGetInfo instead of nvidia-smi.
+"""
+from __future__ import annotations
+
+import gc
+import json
+import math
+import os
+import pickle
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from .packing import pack_matrix_list
+from .parafac2_intergrative import Fast_Higashi_core
+from .sparse_for_schic import Chrom_Dataset, Sparse
+
+CPU_per_GPU = 4
+
+
+def get_config(config_path="./config.jSON"):
+    with open(config_path, "r") as f:
+        return json.load(f)
+
+
+def get_free_gpu(num=1):
+    """(device, id, free bytes) of the device with the most free memory."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("FastHigashi needs a gfx950 device: no HIP device is visible")
+    free = [torch.cuda.mem_get_info(i)[0] for i in range(torch.cuda.device_count())]
+    gid = int(np.argmax(free))
+    torch.cuda.set_device(gid)
+    print("setting to gpu:%d" % gid, "available memory =", free[gid] // 1000000, "MB")
+    return torch.device("cuda:%d" % gid), gid, free[gid]
+
+
+def parse_embedding(project_list, fac, dim=None):
+    """FastHigashi_Wrapper.py:92-110."""
+    from sklearn.decomposition import TruncatedSVD
+    dim = fac.shape[1] if dim is None else min(dim, fac.shape[1])
+    parts = []
+    for p in project_list:
+        p = p.detach().cpu().numpy() if isinstance(p, torch.Tensor) else p
+        parts.append(fac @ (p / np.linalg.norm(p, axis=0, keepdims=True)))
+    return TruncatedSVD(n_components=dim).fit_transform(np.concatenate(parts, axis=1))
+
+
+class FastHigashi:
+    def __init__(self, config_path, path2input_cache, path2result_dir, off_diag, filter, do_conv, do_rwr, do_col,
+                 no_col):
+        self.off_diag, self.filter = off_diag, filter
+        self.do_conv, self.do_rwr, self.do_col, self.no_col = do_conv, do_rwr, do_col, no_col
+        self.config_path = config_path
+        self.config = get_config(config_path)
+        self.chrom_list = self.config["chrom_list"]
+        self.temp_dir = self.config["temp_dir"]
+        self.data_dir = self.config["data_dir"]
+        self.fh_resolutions = self.config["resolution_fh"]
+        self.embedding_storage = None
+        self.model = None
+        path2input_cache = self.temp_dir if path2input_cache is None else path2input_cache
+        path2result_dir = self.temp_dir if path2result_dir is None else path2result_dir
+        os.makedirs(path2input_cache, exist_ok=True)
+        os.makedirs(path2result_dir, exist_ok=True)
+        self.path2input_cache, self.path2result_dir = str(path2input_cache), str(path2result_dir)
+        _, self.gpu_id, self.avail_mem = get_free_gpu()
+        self.device = "cuda"
+        torch.set_num_threads(CPU_per_GPU)
+
+    def fast_process_data(self):
+        raise NotImplementedError("contact-pair preprocessing (Fast_process.py) is outside this package: "
+                                  "provide temp_dir/raw/<chrom>_sparse_adj.npy")
+
+    # ------------------------------------------------------------------ meta
+    def _raw(self, chrom):
+        # the reference's own intermediate format: an object array of scipy matrices
+        return np.load(os.path.join(self.temp_dir, "raw", "%s_sparse_adj.npy" % chrom), allow_pickle=True)
+
+    def preprocess_meta(self):
+        """FastHigashi_Wrapper.py:176-211: QC, good cells first, label table in that order."""
+        import pandas as pd
+        qc_path = os.path.join(self.path2input_cache, "qc.npy")
+        if os.path.isfile(qc_path):
+            qc = np.load(qc_path)
+            readcount = np.load(os.path.join(self.path2input_cache, "read_count_all.npy"))
+        else:
+            qc, readcount = self.get_qc()
+        good_qc, bad_qc = np.where(qc > 0)[0], np.where(qc <= 0)[0]
+        reorder = np.concatenate([np.sort(good_qc), np.sort(bad_qc)], axis=0)
+        label_path = os.path.join(self.data_dir, "label_info.pickle")
+        if os.path.isfile(label_path):
+            with open(label_path, "rb") as f:
+                label_info = pd.DataFrame(pickle.load(f))
+        else:
+            label_info = pd.DataFrame()
+        if len(label_info) == 0:
+            label_info = pd.DataFrame(np.ones(len(readcount)), columns=["placeholder"])
+        plot_label = self.config.get("plot_label", [])
+        np.save(os.path.join(self.path2input_cache, "reorder.npy"), reorder)
+        label_info = label_info.iloc[reorder].reset_index()
+        sig_list = label_info[plot_label]
+        if "batch_id" in self.config:
+            self.batch_id = np.asarray(label_info[self.config["batch_id"]])
+        np.save(qc_path, qc)
+        np.save(os.path.join(self.path2input_cache, "read_count_all.npy"), readcount)
+        return label_info, reorder, sig_list, readcount, qc
+
+    @staticmethod
+    def sum_sparse(m):
+        x = np.zeros(m[0].shape)
+        for a in m:
+            x[np.repeat(np.arange(a.shape[0]), np.diff(a.indptr)), a.indices] += a.data
+        return x
+
+    @staticmethod
+    def sum_sparse_coo(m):
+        x = np.zeros(m[0].shape)
+        for a in m:
+            x[a.row, a.col] += a.data
+        return x
+
+    def sum_sparse_by_batch_id(self, m_list):
+        out = {b: np.zeros(m_list[0].shape) for b in np.unique(self.batch_id)}
+        for a, b in zip(m_list, self.batch_id):
+            out[b][a.row, a.col] += a.data
+        return out
+
+    def get_qc(self):
+        """FastHigashi_Wrapper.py:428-458: a cell passes when its contact count beats the
+        chromosome's covered-bin count (or the median) on every chromosome."""
+        scale = int(1000000 / self.config["resolution"])
+        mask, read_count_all = [], 0
+        for chrom in self.chrom_list:
+            a = self._raw(chrom)
+            bulk = self.sum_sparse(a)
+            cov = np.sum(bulk > 0, axis=-1)
+            n_bin = np.sum(cov > 0.1 * cov.shape[0] * scale)
+            m_c = np.array([(len(m.data) + np.sum(m.diagonal() > 0)) / 2 for m in a]).astype("float")
+            rc = np.asarray([np.sum(m) for m in a])
+            if np.sum(m_c > n_bin) > 0.5 * len(m_c):
+                m_c = m_c > n_bin
+            else:
+                m_c = m_c > np.quantile(m_c, 0.5)
+            mask.append(m_c)
+            read_count_all = read_count_all + rc
+        kept = (np.sum(np.array(mask).astype("float"), axis=0) >= len(self.chrom_list)).astype("float32")
+        return kept, np.log1p(read_count_all)
+
+    # ------------------------------------------------------------------ packing
+    def pack_training_data_one_process(self, raw_dir, chrom, reorder, off_diag=None, fac_size=None,
+                                       merge_fac_row=1, merge_fac_col=1, is_sym=True,
+                                       filename_pattern="%s_sparse_adj.npy", force_shift=None, batch_norm=True,
+                                       bar=None):
+        """FastHigashi_Wrapper.py:221-366 (fac_size 1, symmetric)."""
+        if fac_size not in (None, 1) or not is_sym or merge_fac_row != merge_fac_col:
+            raise NotImplementedError("only fac_size=1, symmetric, square merge is used by prep_dataset")
+        a = np.load(os.path.join(raw_dir, filename_pattern % chrom), allow_pickle=True)[reorder]
+        bl_path = os.path.join(self.temp_dir, "raw", "blacklist.npy")
+        blacklist = np.load(bl_path, allow_pickle=True).item()[chrom] if os.path.isfile(bl_path) else None
+        batch = self.batch_id if "batch_id" in self.config else None
+        return pack_matrix_list(list(a), off_diag, merge_fac_row, blacklist, batch, batch_norm)
+
+    def preprocess_contact_map(self, config, reorder, path2input_cache, batch_norm, key_fn=lambda c: c, **kwargs):
+        """Packed + CSR-sorted Sparse per chromosome, cached at path2input_cache (.npz)."""
+        cache = str(path2input_cache)
+        if not cache.endswith(".npz"):
+            cache = os.path.splitext(cache)[0] + ".npz"
+        if os.path.exists(cache):
+            print(f"loading cached input from {cache}")
+            z = np.load(cache)
+            out = []
+            for i, _ in enumerate(self.chrom_list):
+                s = Sparse(z["indices_%d" % i], z["values_%d" % i], tuple(z["shape_%d" % i]), copy=False)
+                s.sort_indices()
+                out.append(s)
+            return out
+        print(f"saving cached input to {cache}")
+        store, out = {}, []
+        for i, chrom in enumerate(config["chrom_list"]):
+            indices, values, shape = self.pack_training_data_one_process(
+                raw_dir=Path(config["temp_dir"]) / "raw", chrom=chrom, reorder=reorder, batch_norm=batch_norm,
+                **kwargs)
+            obj = Sparse(indices, values, shape, copy=False)
+            obj.sort_indices()
+            store["indices_%d" % i], store["values_%d" % i] = obj.indices, obj.values
+            store["shape_%d" % i] = np.asarray(obj.shape)
+            out.append(obj)
+        np.savez(cache, **store)
+        return out
+
+    def prep_dataset(self, meta_only=False, batch_norm=True):
+        """FastHigashi_Wrapper.py:460-567: pack, tile (bs_bin, bs_cell), pick do_col."""
+        self.label_info, reorder, self.sig_list, readcount, qc = self.preprocess_meta()
+        self.reorder = reorder
+        self.coverage_feats = readcount[reorder].reshape((-1, 1))
+        if meta_only:
+            return
+        good_qc_num = int(np.sum(qc > 0))
+        print("total number of cells that pass qc check", good_qc_num, "bad", len(qc) - good_qc_num,
+              "total:", len(qc))
+        tensor_list, recommend_bs_cell = [], []
+        for res in self.fh_resolutions:
+            cache = os.path.join(self.path2input_cache, "cache_intra_%d_offdiag_%d_%s.npz" % (res, self.off_diag, ""))
+            fac = int(res / self.config["resolution"])
+            all_matrix = self.preprocess_contact_map(self.config, reorder=reorder, path2input_cache=cache,
+                                                     batch_norm=batch_norm, is_sym=True, off_diag=self.off_diag,
+                                                     fac_size=1, merge_fac_row=fac, merge_fac_col=fac,
+                                                     filename_pattern="%s_sparse_adj.npy", force_shift=False)
+            num_cell = int(all_matrix[-1].shape[-1])
+            max_tensor_size = self.avail_mem / (4 * 12)
+            rec_bs_bin = min(max(int(15000000 / res), 128), 256)
+            n_cells_fit = good_qc_num if self.filter else num_cell
+            total_reads, total_possible = 0, 0
+            for i, m in enumerate(all_matrix):
+                size = int(m.shape[0])
+                n_batch = max(math.ceil(size / rec_bs_bin), 1)
+                bs_bin = math.ceil(size / n_batch)
+                bs_cell = max(int(max_tensor_size / (bs_bin * (bs_bin + 2 * self.off_diag))), 1)
+                n_batch = int(math.ceil(n_cells_fit / bs_cell))
+                bs_cell = min(int(math.ceil(n_cells_fit / n_batch)), n_cells_fit)
+                recommend_bs_cell.append(bs_cell)
+                total_reads += len(m.values)
+                total_possible += int(np.prod(np.asarray(m.shape, dtype=np.float64)))
+                tensor_list.append(Chrom_Dataset(tensor=m, bs_bin=bs_bin, bs_cell=bs_cell,
+                                                 good_qc_num=good_qc_num if self.filter else -1, kind="hic",
+                                                 upper_sim=False, compact=True, flank=self.off_diag,
+                                                 chrom=self.chrom_list[i], resolution=res))
+            sparsity = total_reads / total_possible
+            print("sparsity", sparsity)
+            do_col = sparsity * (500000 / res) ** 2 <= 0.03 or self.do_col          # :540-553
+            if self.no_col:
+                do_col = False
+            print("do_conv", self.do_conv, "do_rwr", self.do_rwr, "do_col", do_col)
+            self.final_do_col = do_col
+            if self.no_col and self.do_col:
+                print("choose one between do col or no col!")
+                raise EOFError
+        print("recommend_bs_cell", recommend_bs_cell)
+        gc.collect()
+        self.good_qc_num = good_qc_num
+        self.all_matrix = tensor_list
+
+    # ------------------------------------------------------------------ model
+    def run_model(self, dim1=.6, rank=256, n_iter_parafac=1, n_iter_max=None, tol=2e-5, extra="", run_init=True):
+        """FastHigashi_Wrapper.py:657-701."""
+        self.rank = rank
+        save_str = "dim1_%.1f_rank_%d_niterp_%d_%s" % (dim1, rank, n_iter_parafac, extra)
+        self.save_str = save_str
+        print(save_str)
+        start = time.time()
+        if self.model is None:
+            self.model = Fast_Higashi_core(rank=rank, off_diag=self.off_diag, res_list=self.fh_resolutions).to(
+                self.device)
+        if n_iter_max is None:
+            n_iter_max = int(self.good_qc_num / 15)
+        _, factors_all, p_list = self.model.fit_transform(
+            self.all_matrix, size_ratio=dim1, n_iter_max=n_iter_max, n_iter_parafac=n_iter_parafac,
+            do_conv=self.do_conv, do_rwr=self.do_rwr, do_col=self.final_do_col, tol=tol, gpu_id=self.gpu_id,
+            run_init=run_init)
+        print("takes: %.2f s" % (time.time() - start))
+        A_list, B_list, D_list, meta_embedding = factors_all
+        self.meta_embedding = meta_embedding.detach().cpu().numpy()
+        self.A_list = [A.detach().cpu().numpy() for A in A_list]
+        self.B_list = [B.detach().cpu().numpy() for B in B_list]
+        self.D_list = [D.detach().cpu().numpy() for D in D_list]
+        self.p_list = [[p.detach().cpu().numpy() for p in t] for t in p_list]
+        with open(os.path.join(self.path2result_dir, "results_all%s.pkl" % save_str), "wb") as f:
+            pickle.dump([self.A_list, self.B_list, self.D_list, self.meta_embedding, self.p_list], f, protocol=4)
+        with open(os.path.join(self.path2result_dir, "results%s.pkl" % save_str), "wb") as f:
+            pickle.dump([self.meta_embedding, self.D_list], f, protocol=4)
+
+    def load_model(self, dim1=.6, rank=256, n_iter_parafac=1, extra=""):
+        """Results written by run_model of this package."""
+        save_str = "dim1_%.1f_rank_%d_niterp_%d_%s" % (dim1, rank, n_iter_parafac, extra)
+        with open(os.path.join(self.path2result_dir, "results_all%s.pkl" % save_str), "rb") as f:
+            data = pickle.load(f)
+        print("model loaded")
+        self.A_list, self.B_list, self.D_list, self.meta_embedding, self.p_list = data
+
+    def only_partial_rwr(self):
+        """FastHigashi_Wrapper.py:569-655: per-cell imputed maps (symmetrised, diagonal halved)
+        into path2result_dir/impute_prwr.hdf5 (or impute_prwr/<chrom>.npz without h5py)."""
+        from .partial_rwr import partial_rwr
+        try:
+            import h5py
+            h5 = h5py.File(os.path.join(self.path2result_dir, "impute_prwr.hdf5"), "w")
+        except ImportError:
+            h5 = None
+            os.makedirs(os.path.join(self.path2result_dir, "impute_prwr"), exist_ok=True)
+        for ci, cd in enumerate(self.all_matrix):
+            maps = {}
+            for good, n_batch, off in ((True, cd.num_cell_batch, 0), (False, cd.num_cell_batch_bad, cd.num_cell_batch)):
+                for k in range(n_batch):
+                    cells = cd.cell_slice_list[k + off]
+                    imputed = None
+                    for b in range(cd.num_bin_batch):
+                        sl, sc, sr = cd.local_bin_slice_list[b], cd.col_bin_slice_list[b], cd.bin_slice_list[b]
+                        (x, _), kind = cd.fetch(b, k, save_context=dict(device=self.device), transpose=True,
+                                                do_conv=False, good_qc=good)
+                        if imputed is None:
+                            imputed = np.zeros((int(x.shape[0]), cd.num_bin, cd.num_bin))
+                        x, _ = partial_rwr(x, slice_start=sl.start, slice_end=sl.stop, do_conv=self.do_conv,
+                                           do_rwr=self.do_rwr, do_col=False, return_rwr_iter=True,
+                                           force_rwr_epochs=-1, final_transpose=False)
+                        imputed[:, sr, sc] = x.detach().cpu().numpy()
+                    imputed = imputed + imputed.transpose(0, 2, 1)
+                    for i in range(len(imputed)):
+                        m = imputed[i]
+                        maps[str(self.reorder[cells][i])] = (m - np.diag(np.diag(m) / 2)).astype("float32")
+            chrom = self.chrom_list[ci]
+            if h5 is not None:
+                g = h5.create_group(chrom)
+                g.create_dataset("shape", data=np.asarray([cd.num_bin, cd.num_bin]))
+                for k, v in maps.items():
+                    g.create_dataset(k, data=v)
+            else:
+                np.savez(os.path.join(self.path2result_dir, "impute_prwr", "%s.npz" % chrom),
+                         shape=np.asarray([cd.num_bin, cd.num_bin]), **maps)
+        if h5 is not None:
+            h5.close()
+
+    # ------------------------------------------------------------------ embedding
+    def restore_order_fun(self, x):
+        new_x = np.zeros_like(x)
+        new_x[self.reorder] = x
+        return new_x
+
+    def fetch_cell_embedding(self, final_dim=None, restore_order=False):
+        """FastHigashi_Wrapper.py:750-789: M D_c (column-normalised D) per chromosome,
+        concatenated, TruncatedSVD to final_dim, l2 rows; then the coverage regression."""
+        from sklearn.decomposition import TruncatedSVD
+        from sklearn.preprocessing import normalize, quantile_transform
+        print("fetching embedding")
+        final_dim = self.rank if final_dim is None else final_dim
+        parts = []
+        for p in self.D_list:
+            p = p.detach().cpu().numpy() if isinstance(p, torch.Tensor) else p
+            parts.append(self.meta_embedding @ (p / np.linalg.norm(p, axis=0, keepdims=True)))
+        embedding = np.concatenate(parts, axis=1)
+        self.label_info["coverage_fh"] = quantile_transform(self.coverage_feats, n_quantiles=100)
+        embed = TruncatedSVD(n_components=final_dim).fit_transform(embedding)
+        if restore_order:
+            embedding = self.restore_order_fun(embedding)
+            embed = self.restore_order_fun(embed)
+        store = {"embed_all": embedding, "embed_raw": embed, "embed_l2_norm": normalize(embed),
+                 "restore_order": restore_order}
+        self.embedding_storage = store
+        self.correct_batch_linear("coverage_fh", False)
+        return store
+
+    def _regressor(self, name):
+        v = np.array(self.label_info[name])
+        if self.embedding_storage["restore_order"]:
+            v = self.restore_order_fun(v)
+        if v.dtype not in (np.dtype("float32"), np.dtype("float16"), np.dtype("float64")):
+            print("not float var, one hot encoding")
+            uniq = np.unique(v)
+            v = (v[:, None] == uniq[None, :]).astype(np.float64)
+        return v.reshape((-1, 1)) if v.ndim == 1 else v
+
+    def correct_batch_linear(self, var_to_regress_name, add_intercept_back=False):
+        """FastHigashi_Wrapper.py:815-880: regress covariates out of embed_all, re-reduce."""
+        from sklearn.decomposition import TruncatedSVD
+        from sklearn.linear_model import LinearRegression
+        from sklearn.preprocessing import normalize
+        if self.embedding_storage is None:
+            print("Run fetch_cell_embedding() first!")
+            return None
+        names = [var_to_regress_name] if isinstance(var_to_regress_name, str) else var_to_regress_name
+        if not isinstance(names, list):
+            print("var_to_regress must be a str or list of strs")
+            return None
+        try:
+            var = np.concatenate([self._regressor(n) for n in names], axis=-1)
+        except KeyError:
+            print("var_to_regress not in label_info.pickle!")
+            return None
+        key = "_".join(names)
+        emb = self.embedding_storage["embed_all"]
+        model = LinearRegression().fit(var, emb)
+        emb = emb - model.predict(var)
+        if add_intercept_back:
+            emb = emb + model.intercept_[None]
+        reduce = TruncatedSVD(n_components=self.embedding_storage["embed_raw"].shape[-1]).fit_transform(emb)
+        self.embedding_storage["embed_correct_%s" % key] = reduce
+        self.embedding_storage["embed_l2_norm_correct_%s" % key] = normalize(reduce)
+        return self.embedding_storage

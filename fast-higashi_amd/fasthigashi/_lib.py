@@ -110,7 +110,7 @@ _SIGS = {
     "fh_cpals_tnorm": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "fh_csr_sort_workspace": (c_int64, [c_int64, c_int32, c_int64]),
     "fh_csr_sort": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_void_p, c_void_p, c_void_p,
-                            c_void_p, c_int64, c_void_p]),
+                            c_void_p, c_void_p, c_int64, c_void_p]),
     "fh_band_scatter": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int32, c_int32, c_void_p,
                                 c_int, c_void_p, c_void_p]),
     "fh_transpose": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int, c_int64, c_void_p,
@@ -124,6 +124,8 @@ _SIGS = {
                             c_void_p]),
     "fh_pool": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64,
                         c_void_p]),
+    "fh_avgpool3": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
+                            c_void_p]),
     "fh_colsum_add": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64,
                               c_void_p]),
 }

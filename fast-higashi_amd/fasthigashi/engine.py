@@ -97,6 +97,124 @@ class MultiGemm:
             g.run(stream)
 
 
+class CpAls:
+    """Inner CP-ALS (parafac_integrative.py:28-112) on several 3-way tensors at once.
+
+    items: per tensor a dict with device pointers T ((n*sp) x R fp32, rows i*sp + j),
+    A (n x sp), B (s x sp), D (R x Dld, columns [0, s)) and sizes n, s, sp, Dld.  Every
+    step is one grouped launch over all tensors; the inner loop's stop test and the
+    skipped balance run on the device (one flag per tensor)."""
+
+    def __init__(self, items, R: int, device):
+        dev = torch.device(device)
+        z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
+        f4 = lambda t, o=0: t.data_ptr() + 4 * o
+        self.K = K = len(items)
+        self.R = R
+        self.items = items
+        self.mdsplit = [max(1, min(32, (q["n"] * q["sp"]) // 2048)) for q in items]
+        self.W1 = [z(q["n"] * q["sp"] * q["sp"]) for q in items]
+        self.Gr = [z(4, q["s"], q["sp"], dt=torch.float64) for q in items]   # GA, GB, GD, system matrix
+        self.Hinv = [z(q["s"], q["sp"], dt=torch.float64) for q in items]
+        self.MF = [z(max(q["n"], R) * q["sp"]) for q in items]
+        self.KRf = [z(q["s"] * q["n"] * q["sp"]) for q in items]
+        self.MDpart = [z(k * R * q["sp"]) for k, q in zip(self.mdsplit, items)]
+        self.flags = torch.zeros(K, dtype=torch.int32, device=dev)
+        self.state = z(K, 2, dt=torch.float64)
+        self.scal = z(K, 4, dt=torch.float64)
+        self.core = [z(K, dt=torch.float64), z(K, dt=torch.float64)]
+        self.part_blk = 64
+        self.part = z(K * self.part_blk, dt=torch.float64)
+        self.spd_status = torch.zeros(1, dtype=torch.int32, device=dev)
+        tabs = [[], []]
+        w1, gA, gB, gD, uA, uB, uD, md, spd = [], [], [], [], [], [], [], [], []
+        for c, it in enumerate(items):
+            n, s, sp, Dld = it["n"], it["s"], it["sp"], it["Dld"]
+            Tp, Ap, Bp, Dp = it["T"], it["A"], it["B"], it["D"]
+            skip = self.flags.data_ptr() + 4 * c
+            Gr = self.Gr[c]
+            gp = [Gr[i].data_ptr() for i in range(4)]
+            for k in range(2):
+                q = L.CpChrom()
+                q.A, q.B, q.D, q.T = Ap, Bp, Dp, Tp
+                q.W1, q.MF, q.KR, q.MDpart = f4(self.W1[c]), f4(self.MF[c]), f4(self.KRf[c]), f4(self.MDpart[c])
+                q.G = Gr.data_ptr()
+                q.scal = self.scal.data_ptr() + 8 * 4 * c
+                q.state = self.state.data_ptr() + 8 * 2 * c
+                q.flag = skip
+                q.core = self.core[k].data_ptr() + 8 * c
+                q.n, q.s, q.sp, q.R, q.Dld, q.mdsplit = n, s, sp, R, Dld, self.mdsplit[c]
+                tabs[k].append(q)
+            w1.append(make_desc(Tp, Dp, f4(self.W1[c]), n * sp, s, R, R, 1, Dld, 1, sp, 1))
+            gA.append(make_desc(Ap, Ap, gp[0], s, s, n, 1, sp, sp, 1, sp, 1))
+            gB.append(make_desc(Bp, Bp, gp[1], s, s, s, 1, sp, sp, 1, sp, 1))
+            gD.append(make_desc(Dp, Dp, gp[2], s, s, R, 1, Dld, Dld, 1, sp, 1))
+            MF, Hi = f4(self.MF[c]), self.Hinv[c].data_ptr()
+            uA.append(make_desc(MF, Hi, Ap, n, s, s, sp, 1, 1, sp, sp, 1, skip=skip))
+            uB.append(make_desc(MF, Hi, Bp, s, s, s, sp, 1, 1, sp, sp, 1, skip=skip))
+            uD.append(make_desc(MF, Hi, Dp, R, s, s, sp, 1, 1, sp, Dld, 1, skip=skip))
+            ks = self.mdsplit[c]
+            md.append(make_desc(Tp, f4(self.KRf[c]), f4(self.MDpart[c]), R, s, n * sp, 1, R, 1, n * sp, sp, 1,
+                                flags=L.GEMM_SPLIT_PARTIAL, ksplit=ks, sCk=R * sp))
+            q = L.SpdSys()
+            q.G, q.out, q.s, q.ldg, q.ldo, q.eps = gp[3], self.Hinv[c].data_ptr(), s, sp, sp, 1e-10
+            spd.append(q)
+        self.tab = [ops._upload((L.CpChrom * K)(*t), dev) for t in tabs]
+        self.spd_tab = ops._upload((L.SpdSys * K)(*spd), dev)
+        self.g_w1 = MultiGemm(w1, F32, lambda d: shape_for_n(d.N), dev)
+        self.g_gA = ops.Gemm64Group(gA, F32, F32, F64, dev)
+        self.g_gB = ops.Gemm64Group(gB, F32, F32, F64, dev)
+        self.g_gD = ops.Gemm64Group(gD, F32, F32, F64, dev)
+        self.g_gBD = ops.Gemm64Group(gB + gD, F32, F32, F64, dev)
+        self.g_gABD = ops.Gemm64Group(gA + gB + gD, F32, F32, F64, dev)
+        self.g_uA = ops.Gemm64Group(uA, F32, F64, F32, dev)
+        self.g_uB = ops.Gemm64Group(uB, F32, F64, F32, dev)
+        self.g_uD = ops.Gemm64Group(uD, F32, F64, F32, dev)
+        self.g_md = MultiGemm(md, F32, lambda d: 6 if d.N > 160 else shape_for_n(d.N), dev)
+        self.mx = dict(s2=max(q["s"] ** 2 for q in items), ns=max(q["n"] * q["s"] for q in items),
+                       Rs=max(R * q["s"] for q in items), s=max(q["s"] for q in items),
+                       kr=max(q["s"] * q["n"] * q["sp"] for q in items))
+
+    def core_norms(self, which: int, stream=None):
+        """sum(A^T A o B^T B o D^T D) of every tensor's factors into self.core[which]."""
+        self.g_gABD.run(stream)
+        L.call("fh_cpals_op", self.tab[which].data_ptr(), self.K, 5, 0, 1, ops.stream_handle(stream))
+
+    def run(self, n_iter: int, stream=None):
+        """balance_norm, then n_iter ALS iterations with the reference's early stop."""
+        st = ops.stream_handle(stream)
+        tab, K, mx = self.tab[0].data_ptr(), self.K, self.mx
+        L.call("fh_cpals_tnorm", tab, K, self.part.data_ptr(), self.part_blk, st)
+        L.call("fh_cpals_op", tab, K, 4, 0, mx["s"], st)                       # balance_norm (:65)
+        for _ in range(n_iter):
+            self.g_w1.run(stream)                                              # W1 = T x_3 D
+            self.g_gBD.run(stream)
+            self._solve_mode(0, mx["ns"], self.g_uA, st, stream)
+            self.g_gA.run(stream)
+            self._solve_mode(1, mx["s2"], self.g_uB, st, stream)
+            self.g_gB.run(stream)
+            L.call("fh_cpals_op", tab, K, 0, 2, mx["s2"], st)                  # A^T A o B^T B
+            L.call("fh_spd_inverse", self.spd_tab.data_ptr(), K, mx["s"], self.spd_status.data_ptr(), st)
+            L.call("fh_cpals_op", tab, K, 2, 0, mx["kr"], st)                  # A (.) B
+            self.g_md.run(stream)                                              # T^T (A (.) B), k-split
+            L.call("fh_cpals_op", tab, K, 1, 2, mx["Rs"], st)                  # reduce the partials
+            self.g_uD.run(stream)
+            self.g_gD.run(stream)
+            L.call("fh_cpals_op", tab, K, 3, 0, 1, st)                         # stopping test (:80-107)
+            L.call("fh_cpals_op", tab, K, 4, 1, mx["s"], st)                   # balance unless stopped
+
+    def _solve_mode(self, mode: int, elems: int, upd, st, stream):
+        tab, K, mx = self.tab[0].data_ptr(), self.K, self.mx
+        L.call("fh_cpals_op", tab, K, 0, mode, mx["s2"], st)                   # Gram Hadamard
+        L.call("fh_spd_inverse", self.spd_tab.data_ptr(), K, mx["s"], self.spd_status.data_ptr(), st)
+        L.call("fh_cpals_op", tab, K, 1, mode, elems, st)                      # MTTKRP from W1
+        upd.run(stream)
+
+    def loss_terms(self):
+        """(||T||^2, <M_D, D>, ||[[A,B,D]]||^2) per tensor after the last iteration (host)."""
+        return self.scal[:, :3].cpu().numpy()
+
+
 class SweepEngine:
     """ALS sweeps over a fixed dataset resident in HBM.
 
@@ -171,25 +289,9 @@ class SweepEngine:
         self.pb = ops.PolarBatch([c.s for c in self.chroms], [c.n for c in self.chroms], ns_iter, ns_tol, dev)
         self.pbm = ops.PolarBatch([self.R], [1], ns_iter, ns_tol, dev)
         self.zbuf = ns_iter & 1       # buffer index holding the result (fixed launch count)
-        # CP-ALS per-chromosome buffers
-        smax = max(c.s for c in self.chroms)
-        spmax = max(c.sp for c in self.chroms)
-        nspmax = max(c.n * c.sp for c in self.chroms)
-        self.W1 = [z(c.n * c.sp * c.sp) for c in self.chroms]
-        self.Gr = [z(4, c.s, c.sp, dt=torch.float64) for c in self.chroms]   # GA, GB, GD, H
-        self.Hinv = [z(c.s, c.sp, dt=torch.float64) for c in self.chroms]
-        self.MF = [z(max(c.n, self.R) * c.sp) for c in self.chroms]         # MTTKRP results
-        self.KRf = [z(c.s * c.n * c.sp) for c in self.chroms]               # fp32 Khatri-Rao for M_D
-        self.mdsplit = [max(1, min(32, (c.n * c.sp) // 2048)) for c in self.chroms]
-        self.MDpart = [z(k * self.R * c.sp) for k, c in zip(self.mdsplit, self.chroms)]
-        self.flags = torch.zeros(self.K, dtype=torch.int32, device=dev)
-        self.cp_state = z(self.K, 2, dt=torch.float64)
-        self.cp_scal = z(self.K, 4, dt=torch.float64)       # tnorm2, inner, xhat2, spare
-        self.spd_status = torch.zeros(1, dtype=torch.int32, device=dev)
         # scalars
         self.tnorm = z(self.K, dt=torch.float64)
         self.xU = z(self.K, dt=torch.float64)
-        self.core = [z(self.K, dt=torch.float64), z(self.K, dt=torch.float64)]
         self.core_cur = 0
         self.hist_w = 3 + 3 * self.K
         self.hist = z(n_iter_max_hist, self.hist_w, dt=torch.float64)
@@ -198,6 +300,7 @@ class SweepEngine:
         self.iteration = 0
         self.n_iter_parafac = 1
         self.timing = None         # set to [] to record per-phase HIP events
+        self.spd_status = torch.zeros(1, dtype=torch.int32, device=dev)
         self._build()
 
     # ------------------------------------------------------------------ views
@@ -371,64 +474,15 @@ class SweepEngine:
         self._build_cpals()
 
     def _build_cpals(self):
-        dev, R, Dld, K = self.device, self.R, self.Dld, self.K
         f4 = lambda t, o=0: t.data_ptr() + 4 * o
-        self.cp_part_blk = 64
-        self.cp_part = torch.zeros(K * self.cp_part_blk, dtype=torch.float64, device=dev)
-        tabs = [[], []]
-        w1, gA, gB, gD, uA, uB, uD, md, spd = [], [], [], [], [], [], [], [], []
-        for c, cl in enumerate(self.chroms):
-            n, s, sp = cl.n, cl.s, cl.sp
-            skip = self.flags.data_ptr() + 4 * c
-            Ap, Bp, Dp = f4(self.A_all, cl.aoff), f4(self.B_all, cl.boff), f4(self.D_all, cl.doff)
-            Gr = self.Gr[c]
-            gp = [Gr[i].data_ptr() for i in range(4)]
-            for k in range(2):
-                q = L.CpChrom()
-                q.A, q.B, q.D, q.T = Ap, Bp, Dp, f4(self.Ttil, cl.ttoff)
-                q.W1, q.MF, q.KR, q.MDpart = f4(self.W1[c]), f4(self.MF[c]), f4(self.KRf[c]), f4(self.MDpart[c])
-                q.G = Gr.data_ptr()
-                q.scal = self.cp_scal.data_ptr() + 8 * 4 * c
-                q.state = self.cp_state.data_ptr() + 8 * 2 * c
-                q.flag = skip
-                q.core = self.core[k].data_ptr() + 8 * c
-                q.n, q.s, q.sp, q.R, q.Dld, q.mdsplit = n, s, sp, R, Dld, self.mdsplit[c]
-                tabs[k].append(q)
-            w1.append(make_desc(f4(self.Ttil, cl.ttoff), Dp, f4(self.W1[c]), n * sp, s, R, R, 1, Dld, 1, sp, 1))
-            gA.append(make_desc(Ap, Ap, gp[0], s, s, n, 1, sp, sp, 1, sp, 1))
-            gB.append(make_desc(Bp, Bp, gp[1], s, s, s, 1, sp, sp, 1, sp, 1))
-            gD.append(make_desc(Dp, Dp, gp[2], s, s, R, 1, Dld, Dld, 1, sp, 1))
-            MF, Hi = f4(self.MF[c]), self.Hinv[c].data_ptr()
-            uA.append(make_desc(MF, Hi, Ap, n, s, s, sp, 1, 1, sp, sp, 1, skip=skip))
-            uB.append(make_desc(MF, Hi, Bp, s, s, s, sp, 1, 1, sp, sp, 1, skip=skip))
-            uD.append(make_desc(MF, Hi, Dp, R, s, s, sp, 1, 1, sp, Dld, 1, skip=skip))
-            ks = self.mdsplit[c]
-            md.append(make_desc(f4(self.Ttil, cl.ttoff), f4(self.KRf[c]), f4(self.MDpart[c]), R, s, n * sp,
-                                1, R, 1, n * sp, sp, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=ks, sCk=R * sp))
-            q = L.SpdSys()
-            q.G, q.out, q.s, q.ldg, q.ldo, q.eps = gp[3], self.Hinv[c].data_ptr(), s, sp, sp, 1e-10
-            spd.append(q)
-        self.cp_tab = [ops._upload((L.CpChrom * K)(*t), dev) for t in tabs]
-        self.spd_tab = ops._upload((L.SpdSys * K)(*spd), dev)
-        self.g_w1 = MultiGemm(w1, F32, lambda d: shape_for_n(d.N), dev)
-        self.g_gA = ops.Gemm64Group(gA, F32, F32, F64, dev)
-        self.g_gB = ops.Gemm64Group(gB, F32, F32, F64, dev)
-        self.g_gD = ops.Gemm64Group(gD, F32, F32, F64, dev)
-        self.g_gBD = ops.Gemm64Group(gB + gD, F32, F32, F64, dev)
-        self.g_gABD = ops.Gemm64Group(gA + gB + gD, F32, F32, F64, dev)
-        self.g_uA = ops.Gemm64Group(uA, F32, F64, F32, dev)
-        self.g_uB = ops.Gemm64Group(uB, F32, F64, F32, dev)
-        self.g_uD = ops.Gemm64Group(uD, F32, F64, F32, dev)
-        self.g_md = MultiGemm(md, F32, lambda d: 6 if d.N > 160 else shape_for_n(d.N), dev)
-        self.cp_max = dict(s2=max(c.s * c.s for c in self.chroms), ns=max(c.n * c.s for c in self.chroms),
-                           Rs=max(self.R * c.s for c in self.chroms), s=max(c.s for c in self.chroms),
-                           kr=max(c.s * c.n * c.sp for c in self.chroms))
+        items = [dict(T=f4(self.Ttil, cl.ttoff), A=f4(self.A_all, cl.aoff), B=f4(self.B_all, cl.boff),
+                      D=f4(self.D_all, cl.doff), n=cl.n, s=cl.s, sp=cl.sp, Dld=self.Dld) for cl in self.chroms]
+        self.cpals = CpAls(items, self.R, self.device)
+        self.core = self.cpals.core
 
     # ------------------------------------------------------------------ sweep
     def _core_norms(self, which: int, stream=None):
-        """Core norms sum(A^T A o B^T B o D^T D) of every chromosome into self.core[which]."""
-        self.g_gABD.run(stream)
-        L.call("fh_cpals_op", self.cp_tab[which].data_ptr(), self.K, 5, 0, 1, ops.stream_handle(stream))
+        self.cpals.core_norms(which, stream)
 
     def _mark(self, name, stream):
         if self.timing is not None:
@@ -457,6 +511,24 @@ class SweepEngine:
             if it % 10 == 0 and it > 0 and self.n_iter_parafac < 10:       # :636-637
                 self.n_iter_parafac += 1
             n_iter_parafac = self.n_iter_parafac
+        self.passes(stream)
+        # errors with the previous core norm (:667-668)
+        prev = self.core[self.core_cur]
+        L.call("fh_sweep_errors", self.tnorm.data_ptr(), prev.data_ptr(), self.xU.data_ptr(),
+               self.pbm.nuc.data_ptr(), self.K, self.hist.data_ptr() + 8 * self.hist_w * it, st)
+        # CP-ALS on every T_c, then the new core norm (:674-706)
+        mk("cpals")
+        self._cpals(n_iter_parafac, stream)
+        self.core_cur ^= 1
+        self._core_norms(self.core_cur, stream)
+        mk("end")
+        self.iteration += 1
+
+    def passes(self, stream=None):
+        """update_meta_embedding_interactions (parafac2_intergrative.py:303-540): new P_i, x_U,
+        M (and x_V), T_c."""
+        st = ops.stream_handle(stream)
+        mk = lambda name: self._mark(name, stream)
         # pass 1
         mk("ct")
         self.g_ct.run(stream)
@@ -499,47 +571,41 @@ class SweepEngine:
         # pass 3
         mk("ttilde")
         self.g_ttil.run(stream)
-        # errors with the previous core norm (:667-668)
-        prev = self.core[self.core_cur]
-        L.call("fh_sweep_errors", self.tnorm.data_ptr(), prev.data_ptr(), self.xU.data_ptr(),
-               self.pbm.nuc.data_ptr(), self.K, self.hist.data_ptr() + 8 * self.hist_w * it, st)
-        # CP-ALS on every T_c, then the new core norm (:674-706)
-        mk("cpals")
-        self._cpals(n_iter_parafac, stream)
-        self.core_cur ^= 1
-        self._core_norms(self.core_cur, stream)
-        mk("end")
-        self.iteration += 1
+
+    def transform(self, stream=None) -> np.ndarray:
+        """Final meta-embedding (parafac2_intergrative.py:742-834): SVD_term from the stored
+        projections and the current A, B, D, then its polar factor.  Leaves self.M unchanged."""
+        st = ops.stream_handle(stream)
+        M_keep = self.M.clone()
+        if self.dtype == "bf16":
+            L.call("fh_copy2d", self.P.data_ptr(), F32, self.P.numel(), 1, self.P16.data_ptr(), BF16,
+                   self.P.numel(), 1, self.P.numel(), 1.0, st)
+        self.g_that.run(stream)
+        for c, cl in enumerate(self.chroms):
+            L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,
+                   cl.sp, self.Lk.data_ptr() + self.Lk.element_size() * cl.loff, self.xdt, st)
+        self.g_z.run(stream)
+        self.g_svd.run(stream)
+        self.g_gm.run(stream)
+        L.call("fh_batch_sum", self.GMpart.data_ptr(), F64, self.meta_ksplit, self.R * self.R, self.R * self.R,
+               self.Gm.data_ptr(), st)
+        self.pbm.solve(stream)
+        self.g_m.run(stream)
+        out = self.M[:self.N].cpu().numpy().copy()
+        self.M.copy_(M_keep)
+        return out
+
+    def set_M(self, M: np.ndarray):
+        self.M.zero_()
+        self.M[:self.N] = torch.as_tensor(np.asarray(M, dtype=np.float32), device=self.device)
+
+    def projected_tensor(self, c: int) -> np.ndarray:
+        """T_c (n, s, R) of the last sweep (the reference's projected_tensor_list entry)."""
+        cl = self.chroms[c]
+        return self.T_view(c)[:, :cl.s, :].cpu().numpy().copy()
 
     def _cpals(self, n_iter: int, stream=None):
-        """parafac() on every chromosome (parafac_integrative.py:28-112), n_iter inner iterations."""
-        st = ops.stream_handle(stream)
-        tab, K, mx = self.cp_tab[0].data_ptr(), self.K, self.cp_max
-        L.call("fh_cpals_tnorm", tab, K, self.cp_part.data_ptr(), self.cp_part_blk, st)
-        L.call("fh_cpals_op", tab, K, 4, 0, mx["s"], st)                       # balance_norm (:65)
-        for _ in range(n_iter):
-            self.g_w1.run(stream)                                              # W1 = T x_3 D
-            self.g_gBD.run(stream)
-            self._solve_mode(0, mx["ns"], self.g_uA, st, stream)
-            self.g_gA.run(stream)
-            self._solve_mode(1, mx["s2"], self.g_uB, st, stream)
-            self.g_gB.run(stream)
-            L.call("fh_cpals_op", tab, K, 0, 2, mx["s2"], st)                  # A^T A o B^T B
-            L.call("fh_spd_inverse", self.spd_tab.data_ptr(), K, mx["s"], self.spd_status.data_ptr(), st)
-            L.call("fh_cpals_op", tab, K, 2, 0, mx["kr"], st)                  # A (.) B
-            self.g_md.run(stream)                                              # T^T (A (.) B), k-split
-            L.call("fh_cpals_op", tab, K, 1, 2, mx["Rs"], st)                  # reduce the partials
-            self.g_uD.run(stream)
-            self.g_gD.run(stream)
-            L.call("fh_cpals_op", tab, K, 3, 0, 1, st)                         # stopping test (:80-107)
-            L.call("fh_cpals_op", tab, K, 4, 1, mx["s"], st)                   # balance unless stopped
-
-    def _solve_mode(self, mode: int, elems: int, upd, st, stream):
-        tab, K, mx = self.cp_tab[0].data_ptr(), self.K, self.cp_max
-        L.call("fh_cpals_op", tab, K, 0, mode, mx["s2"], st)                   # Gram Hadamard
-        L.call("fh_spd_inverse", self.spd_tab.data_ptr(), K, mx["s"], self.spd_status.data_ptr(), st)
-        L.call("fh_cpals_op", tab, K, 1, mode, elems, st)                      # MTTKRP from W1
-        upd.run(stream)
+        self.cpals.run(n_iter, stream)
 
     # ------------------------------------------------------------------ results
     def history(self, n: Optional[int] = None):

@@ -275,10 +275,10 @@ class PolarBatch:
         self.result = r.value
 
 
-def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24):
+def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24, return_gram: bool = False):
     """Polar factor of a batch of fp32 matrices (b, m, s), m >= s, via FP64 Gram + Newton-Schulz.
 
-    Returns (P fp32 (b, m, s), nuclear norms fp64 (b,))."""
+    Returns (P fp32 (b, m, s), nuclear norms fp64 (b,)[, FP64 Grams (b, s, ld)])."""
     squeeze = Y.dim() == 2
     if squeeze:
         Y = Y.unsqueeze(0)
@@ -289,6 +289,7 @@ def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24):
     d = make_desc(Yc.data_ptr(), Yc.data_ptr(), G.data_ptr(), s, s, m, 1, s, s, 1, G.stride(1), 1, b,
                   m * s, m * s, G.stride(0))
     Gemm64Group([d], DT_CODE[Yc.dtype], DT_CODE[Yc.dtype], L.F64).run()
+    Gcopy = G.clone() if return_gram else None
     pb.solve_fused()
     Z = pb.Zinv(0)
     P = torch.empty((b, m, s), dtype=torch.float32, device=Y.device)
@@ -296,6 +297,8 @@ def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24):
                    Z.stride(0), m * s)
     Gemm64Group([d2], DT_CODE[Yc.dtype], L.F64, L.F32).run()
     nuc = pb.nuc_of(0).clone()
+    if return_gram:
+        return (P[0], nuc[0], Gcopy[0]) if squeeze else (P, nuc, Gcopy)
     if squeeze:
         return P[0], nuc[0]
     return P, nuc

@@ -83,6 +83,118 @@ def pack_chrom(cell, a, b, count, n_bins: int, n_cells: int, off_diag: int):
     return idx, vals, (n_valid, n_valid, n_cells), mapping
 
 
+def valid_bin_mapping(bulk: np.ndarray):
+    """Bin filter of preprocessing.py:474-490 (symmetric): valid <=> bulk row sum > min(0, 0.01 n);
+    returns (mapping with -1 for dropped bins, number of valid bins)."""
+    c = bulk.sum(1)
+    v = c > min(0.0, 0.01 * bulk.shape[1])
+    m = np.cumsum(v) - 1
+    m[~v] = -1
+    return m, int(v.sum())
+
+
+def _diag_profile(m: np.ndarray, k_max: int) -> np.ndarray:
+    """sum of the k-th diagonal, doubled for k > 0, k < k_max (preprocessing.py:245-250)."""
+    out = np.zeros(k_max)
+    for k in range(k_max):
+        out[k] = np.diagonal(m, k).sum() * (1 if k == 0 else 2)
+    return out
+
+
+def batch_normalize(bulk: np.ndarray, batch_bulk, matrix_list, batch_id, off_diag: int):
+    """Per-batch distance and coverage normalisation (preprocessing.py:232-292, norm2 :195-215)."""
+    bk_cov = bulk.sum(axis=-1)
+    bulk = bulk / (np.sqrt(bk_cov[None]) + 1e-15)
+    bulk = bulk / (np.sqrt(bk_cov[:, None]) + 1e-15)
+    bulk_ratio = _diag_profile(bulk, off_diag) / bulk.sum()
+    ratio_of, cov_of = {}, {}
+    for b, mb in batch_bulk.items():
+        m_cov = mb.sum(axis=-1)
+        mm = mb / (np.sqrt(m_cov[None]) + 1e-15)
+        mm = mm / (np.sqrt(m_cov[:, None]) + 1e-15)
+        ratio_of[b] = (_diag_profile(mm, off_diag) / (mm.sum() + 1e-15)) / (bulk_ratio + 1e-15)
+        cov_of[b] = np.array(m_cov)
+    for i, m in enumerate(matrix_list):
+        r, c = m.row, m.col
+        cov, ratio = cov_of[batch_id[i]], ratio_of[batch_id[i]]
+        d = np.abs(r - c).astype(int)
+        data = m.data / (np.sqrt(cov[r]) * np.sqrt(cov[c])) * (np.sqrt(bk_cov[r]) * np.sqrt(bk_cov[c]))
+        m.data = data / (ratio[d] + 1e-15)
+    return matrix_list
+
+
+def pack_matrix_list(matrix_list, off_diag: int, merge_fac: int = 1, blacklist=None, batch_id=None,
+                     batch_norm: bool = True):
+    """Packed COO of one chromosome from per-cell scipy matrices, in the order
+    FastHigashi.pack_training_data_one_process builds it (FastHigashi_Wrapper.py:233-366,
+    fac_size = 1, symmetric input): blacklist rows/cols zeroed, bins merged by merge_fac, band
+    |col - row| <= off_diag, valid-bin remap from the bulk, optional per-batch normalisation,
+    per-cell coverage normalisation (preprocessing.py:137-142), log1p, clip at mean + 15 std.
+
+    Returns (indices int32 [3, nnz] = (row, col, cell), values float32, shape)."""
+    from scipy.sparse import csr_matrix
+    if blacklist is not None and len(blacklist) > 0:                      # :241-251
+        bl = np.asarray(blacklist)
+        bl = bl[bl < matrix_list[0].shape[0]]
+        new = []
+        for m in matrix_list:
+            d = m.astype("float32").toarray()
+            d[bl, :] = 0.0
+            d[:, bl] = 0.0
+            new.append(csr_matrix(d))
+        matrix_list = new
+    mats = [m.tocoo() for m in matrix_list]
+    if merge_fac > 1:                                                     # :255-261
+        for m in mats:
+            m.col //= merge_fac
+            m.row //= merge_fac
+            m.sum_duplicates()
+            m.resize(list(np.ceil(np.array(m.shape) / np.array([merge_fac, merge_fac])).astype("int")))
+    for m in mats:                                                        # :264-269
+        keep = np.abs(m.col - m.row) <= off_diag
+        m.row, m.col, m.data = m.row[keep], m.col[keep], m.data[keep]
+    shape2 = mats[0].shape
+    if batch_id is not None:
+        batch_bulk = {b: np.zeros(shape2) for b in np.unique(batch_id)}
+        for m, b in zip(mats, batch_id):
+            batch_bulk[b][m.row, m.col] += m.data
+        bulk = 0
+        for b in batch_bulk:
+            bulk = bulk + batch_bulk[b]
+    else:
+        bulk = np.zeros(shape2)
+        for m in mats:
+            bulk[m.row, m.col] += m.data
+    mapping, n_valid = valid_bin_mapping(bulk / len(mats))                # :278-281
+    if batch_id is not None and batch_norm:                               # :282-290
+        mats = batch_normalize(bulk, batch_bulk, mats, batch_id, off_diag + 1)
+    for m in mats:                                                        # :292-298
+        m.data *= m.shape[0] / (m.sum() + 1e-15)
+        assert not np.isnan(m.data).any()
+    rows, cols, cells, vals = [], [], [], []
+    for i, m in enumerate(mats):                                          # :312-347
+        if m.nnz == 0:
+            continue
+        r, c = mapping[m.row], mapping[m.col]
+        ok = (r != -1) & (c != -1)
+        rows.append(r[ok])
+        cols.append(c[ok])
+        cells.append(np.full(int(ok.sum()), i, dtype=np.int64))
+        vals.append(np.asarray(m.data[ok], dtype=np.float32))
+    if rows:
+        idx = np.stack([np.concatenate(rows), np.concatenate(cols), np.concatenate(cells)]).astype(np.int32)
+        values = np.concatenate(vals)
+    else:
+        idx, values = np.zeros((3, 0), np.int32), np.zeros(0, np.float32)
+    shape = (n_valid, n_valid, len(mats))
+    assert idx.size == 0 or (idx.min() >= 0 and (idx.max(1) < shape).all())
+    values = np.log1p(values)                                             # :358-364
+    if values.size:
+        mean_, std_ = np.mean(values), np.std(values)
+        values = np.clip(values, a_min=None, a_max=mean_ + 15 * std_)
+    return np.ascontiguousarray(idx), np.ascontiguousarray(values, dtype=np.float32), shape
+
+
 # ---------------------------------------------------------------------------
 # device synthetic packing (benchmark inputs)
 # ---------------------------------------------------------------------------
@@ -98,8 +210,10 @@ class DeviceCOO:
     n_cells: int
 
 
-def csr_sort_device(idx: torch.Tensor, val: torch.Tensor, n_rows: int, chunk: int = 4096):
-    """Stable sort by row on the device (sparse_for_schic.py:133-182 semantics)."""
+def csr_sort_device(idx: torch.Tensor, val: torch.Tensor, n_rows: int, chunk: int = 4096, want_perm=False):
+    """Stable sort by row on the device (sparse_for_schic.py:133-182 semantics).
+
+    Returns (sorted idx, sorted val, indptr[, perm])."""
     nnz = idx.shape[1]
     nnz_chunks = max(1, -(-nnz // chunk))
     while nnz_chunks * n_rows > 64 * 1024 * 1024 and chunk < (1 << 24):
@@ -110,8 +224,11 @@ def csr_sort_device(idx: torch.Tensor, val: torch.Tensor, n_rows: int, chunk: in
     oi = torch.empty_like(idx)
     ov = torch.empty_like(val)
     ptr = torch.empty(n_rows + 1, dtype=torch.int64, device=idx.device)
+    perm = torch.empty(nnz, dtype=torch.int64, device=idx.device) if want_perm else None
     L.call("fh_csr_sort", idx.data_ptr(), val.data_ptr(), nnz, n_rows, chunk, oi.data_ptr(), ov.data_ptr(),
-           ptr.data_ptr(), wsb.data_ptr(), ws, ops.stream_handle())
+           ptr.data_ptr(), perm.data_ptr() if want_perm else None, wsb.data_ptr(), ws, ops.stream_handle())
+    if want_perm:
+        return oi, ov, ptr, perm
     return oi, ov, ptr
 
 
@@ -243,7 +360,6 @@ class TileBuilder:
         hi = int(self.coo.indptr_host[min(tile.start + rows, self.coo.n)])
         nnz = hi - lo
         if nnz > 0:
-            tot = self.coo.idx.shape[1]
             # the kernel reads idx rows at stride nnz: hand it a compact copy of the tile's range
             sub = self.coo.idx[:, lo:hi].contiguous()
             L.call("fh_band_scatter", sub.data_ptr(), self.coo.val.data_ptr() + 4 * lo, nnz,
@@ -253,50 +369,11 @@ class TileBuilder:
 
     def rwr(self, Xc: torch.Tensor, tile, epochs: int, do_col: bool = False,
             bin_cov: Optional[torch.Tensor] = None, cells: Optional[slice] = None):
-        """Partial RWR of a dense tile (partial_rwr.py:83-138).  epochs < 0: auto-stop
-        (max over cells of ||Q_k - Q_{k+1}||_F < 0.01, at most 60 epochs).
-
-        Returns (out (nc, rows, wp) fp32, epoch_count)."""
-        nc, rows, wp = Xc.shape
-        w = tile.width
-        rp = rup(rows, 8)
-        ls = tile.local_start
-        S2 = torch.zeros(nc, rows, rp, dtype=torch.float32, device=self.dev)
-        ops.gemm(Xc, Xc.transpose(1, 2), S2[:, :, :rows])                     # :85
-        Pm = torch.zeros_like(S2)
-        L.call("fh_rwr_prep", S2.data_ptr(), rp, rows * rp, Xc.data_ptr(), wp, rows * wp, ls, rows,
-               Pm.data_ptr(), rp, rows * rp, nc, ops.stream_handle())        # :86-111
-        Q = S2                                                                  # reuse: Q_0 = I
-        ops.fill_(Q, 0.0)
-        L.call("fh_diag", Q.data_ptr(), nc, rows * rp, rp, rows, 1.0, 0, ops.stream_handle())
-        Qn = torch.zeros_like(Q)
-        auto = epochs < 0
-        n_ep = 60 if auto else epochs
-        count = 0
-        delta = torch.empty(nc, dtype=torch.float32, device=self.dev)
-        dmax = torch.empty(1, dtype=torch.float32, device=self.dev)
-        for _ in range(n_ep):                                                  # :115-126
-            ops.gemm(Q[:, :, :rows], Pm[:, :, :rows], Qn[:, :, :rows], alpha=0.5, diag=0.5)
-            if auto:
-                L.call("fh_rwr_delta", Q.data_ptr(), Qn.data_ptr(), rp, rows * rp, rows, nc, delta.data_ptr(),
-                       dmax.data_ptr(), ops.stream_handle())
-                Q, Qn = Qn, Q
-                if float(dmax.item()) < 0.01:
-                    break
-            else:
-                Q, Qn = Qn, Q
-            count += 1
-        A = Xc
-        if do_col:                                                             # :131-135
-            L.call("fh_rwr_symnorm", Q.data_ptr(), Qn.data_ptr(), rp, rows * rp, rows, nc, ops.stream_handle())
-            Q, Qn = Qn, Q
-            n = bin_cov.shape[1]
-            cov = bin_cov[cells.start:cells.stop, tile.col_start: tile.col_start + w]
-            L.call("fh_div_cols", A.data_ptr(), wp, rows * wp, rows, w, cov.data_ptr(), n, nc,
-                   ops.stream_handle())
-        out = torch.zeros(nc, rows, wp, dtype=torch.float32, device=self.dev)
-        ops.gemm(Q[:, :, :rows], A[:, :, :w], out[:, :, :w])                  # :138
-        return out, count
+        """Partial RWR of a dense tile (partial_rwr.py:83-138); see rwr_padded."""
+        cov = None
+        if do_col:
+            cov = bin_cov[cells.start:cells.stop, tile.col_start: tile.col_start + tile.width]
+        return rwr_padded(Xc, tile.width, tile.local_start, epochs, do_col, cov)
 
     def store(self, out: torch.Tensor, tile, X_tile: torch.Tensor, cell0: int, dtype_code: int):
         """Transpose (nc, rows, wp) into the (rows*w, Npad) cell-innermost tile at column cell0."""
@@ -315,9 +392,59 @@ class TileBuilder:
                ops.stream_handle())
 
 
+def rwr_padded(Xc: torch.Tensor, w: int, ls: int, epochs: int, do_col: bool = False,
+               cov: Optional[torch.Tensor] = None):
+    """Partial RWR (partial_rwr.py:83-138) of Xc (nc, rows, wp) fp32 whose columns >= w are
+    zero; [ls, ls + rows) is the diagonal block.  epochs < 0: auto-stop (max over cells of
+    ||Q_k - Q_{k+1}||_F < 0.01, at most 60 epochs).  cov: (nc, w) view of bin_cov for do_col.
+
+    Returns (out (nc, rows, wp) fp32, epoch_count)."""
+    dev = Xc.device
+    nc, rows, wp = Xc.shape
+    rp = rup(rows, 8)
+    S2 = torch.zeros(nc, rows, rp, dtype=torch.float32, device=dev)
+    ops.gemm(Xc, Xc.transpose(1, 2), S2[:, :, :rows])                     # :85
+    Pm = torch.zeros_like(S2)
+    L.call("fh_rwr_prep", S2.data_ptr(), rp, rows * rp, Xc.data_ptr(), wp, rows * wp, ls, rows,
+           Pm.data_ptr(), rp, rows * rp, nc, ops.stream_handle())        # :86-111
+    Q = S2                                                                  # reuse: Q_0 = I
+    ops.fill_(Q, 0.0)
+    L.call("fh_diag", Q.data_ptr(), nc, rows * rp, rp, rows, 1.0, 0, ops.stream_handle())
+    Qn = torch.zeros_like(Q)
+    auto = epochs < 0
+    n_ep = 60 if auto else epochs
+    count = 0
+    delta = torch.empty(nc, dtype=torch.float32, device=dev)
+    dmax = torch.empty(1, dtype=torch.float32, device=dev)
+    for _ in range(n_ep):                                                  # :115-126
+        ops.gemm(Q[:, :, :rows], Pm[:, :, :rows], Qn[:, :, :rows], alpha=0.5, diag=0.5)
+        if auto:
+            L.call("fh_rwr_delta", Q.data_ptr(), Qn.data_ptr(), rp, rows * rp, rows, nc, delta.data_ptr(),
+                   dmax.data_ptr(), ops.stream_handle())
+            Q, Qn = Qn, Q
+            if float(dmax.item()) < 0.01:
+                break
+        else:
+            Q, Qn = Qn, Q
+        count += 1
+    A = Xc
+    if do_col:                                                             # :131-135
+        L.call("fh_rwr_symnorm", Q.data_ptr(), Qn.data_ptr(), rp, rows * rp, rows, nc, ops.stream_handle())
+        Q, Qn = Qn, Q
+        assert cov.stride(1) == 1 and cov.shape == (nc, w) and cov.dtype == torch.float32
+        L.call("fh_div_cols", A.data_ptr(), wp, rows * wp, rows, w, cov.data_ptr(), cov.stride(0), nc,
+               ops.stream_handle())
+    out = torch.zeros(nc, rows, wp, dtype=torch.float32, device=dev)
+    ops.gemm(Q[:, :, :rows], A[:, :, :w], out[:, :, :w])                  # :138
+    return out, count
+
+
 def build_X(engine, coos: Sequence[DeviceCOO], n_i: Sequence[int], do_rwr: bool = True, do_col: bool = False,
-            bin_cov: Optional[Sequence[torch.Tensor]] = None, budget_bytes: int = 8 << 30):
-    """Fill engine.X with the post-RWR tiles of every chromosome (fixed epochs n_i[c])."""
+            bin_cov: Optional[Sequence[torch.Tensor]] = None, budget_bytes: int = 8 << 30, do_conv: bool = False):
+    """Fill engine.X with the post-RWR tiles of every chromosome (fixed epochs n_i[c]).
+
+    X is the same in every sweep (the RWR epochs and bin_cov are fixed after init), so the
+    tiles the reference re-densifies and re-imputes on every fetch are built once."""
     code = engine.xdt
     for c, (cl, coo) in enumerate(zip(engine.chroms, coos)):
         tb = TileBuilder(coo, engine.device, budget_bytes)
@@ -326,6 +453,12 @@ def build_X(engine, coos: Sequence[DeviceCOO], n_i: Sequence[int], do_rwr: bool 
             for k0 in range(0, engine.N, nc_max):
                 cells = slice(k0, min(k0 + nc_max, engine.N))
                 Xc = tb.dense(tl, cells)
+                if do_conv and tl.rows > 1:                                   # partial_rwr.py:77-81
+                    pooled = torch.zeros_like(Xc)
+                    wp = Xc.shape[2]
+                    L.call("fh_avgpool3", Xc.data_ptr(), wp, tl.rows * wp, tl.rows, tl.width, Xc.shape[0],
+                           pooled.data_ptr(), wp, tl.rows * wp, ops.stream_handle())
+                    Xc = pooled
                 if do_rwr:
                     out, _ = tb.rwr(Xc, tl, int(n_i[c]), do_col, bin_cov[c] if bin_cov is not None else None, cells)
                 else:

@@ -1,0 +1,253 @@
+"""Fast_Higashi_core: integrative PARAFAC2 on scHi-C tiles (drop-in for parafac2_intergrative.py).
+
+Same constructor, methods and attributes as the reference.  The sweep loop runs in HBM
+through fasthigashi.engine.SweepEngine; the post-RWR tiles X are built once on the
+device (the reference recomputes them on every fetch: its RWR epochs and bin coverage
+are fixed after init, so every fetch returns the same tile).  Factor naming follows the
+reference: A (bins x s), B (s x s), D (R x s), meta_embedding M (cells x R), C = M D.
+"""
+from __future__ import annotations
+
+import math
+import sys
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import ops
+from .engine import SweepEngine
+from .packing import TileBuilder, build_X
+from .ops import rup
+
+gpu_flag = torch.cuda.is_available()
+
+
+class Fast_Higashi_core:
+
+    def __init__(self, rank, off_diag, res_list):
+        self.rank = rank
+        self.sparse_conv = False
+        self.off_diag = off_diag
+        self.res_list = res_list
+        self.device = torch.device("cpu")
+        self.rec_errors = []
+        self.rec_errors_by_chrom = []
+        self.engine: Optional[SweepEngine] = None
+        self.svd_random_state = None
+        self.path_dtype = "fp32"
+
+    def to(self, device):
+        self.device = torch.device(device)
+        return self
+
+    # ------------------------------------------------------------------ sizes
+    def _sizes(self, schic, size_ratio, size_list):
+        """chrom2size, chrom2num_bin, chrom2id and each dataset's global_slice_bin
+        (parafac2_intergrative.py:558-590)."""
+        rank = self.rank
+        if size_list is None:
+            size_list = [min(int(cd.shape[0] * size_ratio * cd.resolution / 1000000), rank) for cd in schic]
+            chrom2size = {}
+            for cd, s in zip(schic, size_list):
+                chrom2size[cd.chrom] = min(chrom2size.get(cd.chrom, s), s)
+        else:
+            chrom2size = {}
+            for cd, s in zip(schic, size_list):
+                if cd.chrom in chrom2size and chrom2size[cd.chrom] != s:
+                    print("size of the same chromosome must be same!", s, chrom2size[cd.chrom], cd.chrom)
+                    raise EOFError
+                chrom2size[cd.chrom] = s
+        self.chrom2size = chrom2size
+        self.chrom2num_bin = {}
+        self.chrom2id = {c: [] for c in chrom2size}
+        for i, cd in enumerate(schic):
+            self.chrom2id[cd.chrom].append(i)
+            if cd.chrom not in self.chrom2num_bin:
+                self.chrom2num_bin[cd.chrom] = cd.num_bin
+                cd.global_slice_bin = slice(0, cd.num_bin)
+            else:
+                cd.global_slice_bin = slice(self.chrom2num_bin[cd.chrom], self.chrom2num_bin[cd.chrom] + cd.num_bin)
+                self.chrom2num_bin[cd.chrom] += cd.num_bin
+        if any(len(v) > 1 for v in self.chrom2id.values()):
+            raise NotImplementedError("several resolutions of one chromosome (shared B, D across datasets) are "
+                                      "not implemented")
+
+    # ------------------------------------------------------------------ init
+    @torch.no_grad()
+    def init_params(self, schic, do_conv, do_rwr, do_col):
+        """parafac2_intergrative.py:60-301: random A, B; partial-RWR features pooled to 1 Mb,
+        TruncatedSVD per chromosome (host, as the reference), SVD of their concatenation -> M, D;
+        bin coverage and the RWR epoch count n_i of every chromosome."""
+        from sklearn.decomposition import TruncatedSVD
+        rank = self.rank
+        dev = torch.device("cuda")
+        t0 = time.perf_counter()
+        self.A_list = [torch.randn([cd.shape[0], self.chrom2size[cd.chrom]]) * 1e-2 + 1 for cd in schic]
+        self.B_dict = {c: torch.eye(s).add_(torch.randn(s), alpha=1e-2) for c, s in self.chrom2size.items()}
+        bin_cov_list, bad_bin_cov_list, n_i_all, C_blocks = [], [], [], []
+        for cd in schic:
+            if cd.num_cell_batch_bad > 0:
+                raise NotImplementedError("bad-QC cells (good_qc_num < total) are not implemented")
+            N = cd.num_cell
+            bin_cov = torch.full((N, cd.num_bin), 1e-4, dtype=torch.float32, device=dev)
+            ll = int(math.ceil(1000000 / cd.resolution))
+            tb = TileBuilder(cd.coo, dev)
+            n_i_list = []
+            blocks = []
+
+            def run_pass(col_pass):
+                for b, tl in enumerate(cd.tiles):
+                    feat_cells = []
+                    for k, cells in enumerate(cd.cell_slice_list[:cd.num_cell_batch]):
+                        Xc = tb.dense(tl, cells)
+                        if do_conv and tl.rows > 1:
+                            pooled = torch.zeros_like(Xc)
+                            wp = Xc.shape[2]
+                            L.call("fh_avgpool3", Xc.data_ptr(), wp, tl.rows * wp, tl.rows, tl.width, Xc.shape[0],
+                                   pooled.data_ptr(), wp, tl.rows * wp, ops.stream_handle())
+                            Xc = pooled
+                        if do_rwr:
+                            out, n_i = tb.rwr(Xc, tl, -1, col_pass, bin_cov, cells)
+                        else:
+                            out, n_i = Xc, 0
+                        wp = out.shape[2]
+                        if not col_pass:
+                            n_i_list.append(n_i)
+                            cov = bin_cov[cells, tl.col_start: tl.col_start + tl.width]
+                            L.call("fh_colsum_add", out.data_ptr(), wp, tl.rows * wp, tl.rows, tl.width,
+                                   out.shape[0], cov.data_ptr(), cd.num_bin, ops.stream_handle())
+                        if col_pass or not do_col:
+                            pr, pc = tl.rows // ll, tl.width // ll
+                            F = torch.zeros(out.shape[0], pr * pc, dtype=torch.float32, device=dev)
+                            L.call("fh_pool", out.data_ptr(), wp, tl.rows * wp, tl.rows, tl.width, ll, out.shape[0],
+                                   F.data_ptr(), pr * pc, ops.stream_handle())
+                            feat_cells.append(F.cpu().numpy())
+                    if feat_cells:
+                        blocks.append(np.concatenate(feat_cells, axis=0))
+
+            run_pass(False)
+            if do_col:
+                run_pass(True)
+            feats = np.concatenate(blocks, axis=1) if blocks else np.zeros((N, 0), np.float32)
+            size = self.chrom2size[cd.chrom]
+            svd = TruncatedSVD(n_components=size, n_iter=2, random_state=self.svd_random_state)
+            C_blocks.append(svd.fit_transform(feats))
+            n_i_all.append(int(np.max(n_i_list)) if n_i_list else 0)
+            bin_cov[bin_cov <= 1e-4] = float("inf")
+            bin_cov_list.append(bin_cov)
+            bad_bin_cov_list.append(0)
+        self.n_i = np.array(n_i_all)
+        print("rwr iters:", self.n_i)
+        C = np.concatenate(C_blocks, axis=1).astype(np.float64)
+        U, S, Vh = np.linalg.svd(C, full_matrices=False)
+        self.meta_embedding = torch.from_numpy(U[:, :rank].astype(np.float32))
+        SVh = (Vh[:rank] * S[:rank, None]).astype(np.float32)
+        offs = np.concatenate([[0], np.cumsum(list(self.chrom2size.values()))])
+        self.D_dict = {c: torch.from_numpy(SVh[:, a:b].copy())
+                       for c, a, b in zip(self.chrom2size, offs[:-1], offs[1:])}
+        self.bin_cov_list = bin_cov_list
+        self.bad_bin_cov_list = bad_bin_cov_list
+        print(f"time elapsed: {time.perf_counter() - t0:.2f}")
+        print("finish init")
+
+    # ------------------------------------------------------------------ engine
+    def _build_engine(self, schic, do_conv, do_rwr, do_col):
+        tiles = [cd.tile_specs() for cd in schic]
+        sizes = [self.chrom2size[cd.chrom] for cd in schic]
+        N = schic[0].num_cell
+        eng = SweepEngine(tiles, [cd.num_bin for cd in schic], sizes, N, self.rank, dtype=self.path_dtype,
+                          names=[cd.chrom for cd in schic])
+        build_X(eng, [cd.coo for cd in schic], [int(v) for v in self.n_i], do_rwr=do_rwr, do_col=do_col,
+                bin_cov=self.bin_cov_list if do_col else None, do_conv=do_conv)
+        eng.set_factors([a.numpy() for a in self.A_list], [self.B_dict[cd.chrom].cpu().numpy() for cd in schic],
+                        [self.D_dict[cd.chrom].cpu().numpy() for cd in schic],
+                        self.meta_embedding.cpu().numpy())
+        self.engine = eng
+        return eng
+
+    def _pull_factors(self, schic):
+        A, B, D, M = self.engine.get_factors()
+        for i, cd in enumerate(schic):
+            self.A_list[i][:] = torch.from_numpy(A[i])
+            self.B_dict[cd.chrom] = torch.from_numpy(B[i])
+            self.D_dict[cd.chrom] = torch.from_numpy(D[i])
+        self.meta_embedding = torch.from_numpy(M)
+
+    # ------------------------------------------------------------------ sweep
+    @torch.no_grad()
+    def update_meta_embedding_interactions(self, schic, projection_list=None, projected_tensor_list=None,
+                                           do_conv=True, do_rwr=True, do_col=False, first_iter=False):
+        """One pass set of the sweep (parafac2_intergrative.py:303-540) with the current factors.
+        Returns (projection_list, projected_tensor_list, x_U, x_V[, ||X||^2 per chromosome])."""
+        eng = self.engine if self.engine is not None else self._build_engine(schic, do_conv, do_rwr, do_col)
+        eng.passes()
+        torch.cuda.synchronize()
+        x_U = eng.xU.cpu().numpy().reshape(-1, 1)
+        x_V = float(eng.pbm.nuc[0].item())
+        proj = eng.projections()
+        projected = {cd.chrom: torch.from_numpy(eng.projected_tensor(i)) for i, cd in enumerate(schic)}
+        self.meta_embedding = torch.from_numpy(eng.M[:eng.N].cpu().numpy().copy())
+        projection_list = [[torch.from_numpy(p) for p in pl] for pl in proj]
+        if first_iter:
+            return projection_list, projected, x_U, x_V, eng.tnorm.cpu().numpy().reshape(-1, 1)
+        return projection_list, projected, x_U, x_V
+
+    def fit(self, schic, size_ratio=0.3, n_iter_max=2000, n_iter_parafac=5, do_conv=True, do_rwr=False,
+            do_col=False, tol=1e-8, size_list=None, gpu_id=None, verbose=True, run_init=True):
+        """parafac2_intergrative.py:544-740.  Per-iteration fits are kept in self.rec_errors."""
+        self.gpu_id = gpu_id
+        self._sizes(schic, size_ratio, size_list)
+        print("empty params initialized")
+        if run_init:
+            self.init_params(schic, do_conv, do_rwr, do_col)
+        eng = self._build_engine(schic, do_conv, do_rwr, do_col)
+        eng.n_iter_parafac = n_iter_parafac
+        self.rec_errors, self.rec_errors_by_chrom = [], []
+        for iteration in range(n_iter_max):
+            start_time = time.time()
+            eng.sweep()
+            h = eng.hist[iteration].cpu().numpy() if (tol > 0 or verbose) else None
+            if h is None:
+                continue
+            rec = float(h[0])
+            by = h[3::3][: eng.K]
+            self.rec_errors.append(rec)
+            self.rec_errors_by_chrom.append(by)
+            if iteration >= 1:
+                differences = (self.rec_errors_by_chrom[-2] ** 2 - by ** 2) / (self.rec_errors_by_chrom[-2] ** 2)
+                total = (self.rec_errors[-2] ** 2 - rec ** 2) / self.rec_errors[-2] ** 2
+                if verbose:
+                    print(f"PARAFAC2 re={rec:.3f} {total:.2e} variation min{differences.min():.1e} at chrom "
+                          f"{differences.argmin():d}, max{differences.max():.1e} at chrom {differences.argmax():d}",
+                          f"takes {time.time() - start_time:.1f}s")
+                if iteration >= 3 and tol > 0 and (total < tol or differences.max() < tol * 2):
+                    print("converged in {} iterations.".format(iteration))
+                    break
+            elif verbose:
+                print(f"PARAFAC2 re={rec:.3f} takes {time.time() - start_time:.1f}s")
+            sys.stdout.flush()
+        if not self.rec_errors:
+            h = eng.history()
+            self.rec_errors = list(h["rec_error"])
+            self.rec_errors_by_chrom = list(h["rec_error_by_chrom"])
+        self._pull_factors(schic)
+        self.projection_list = [[torch.from_numpy(p) for p in pl] for pl in eng.projections()]
+        self.projected_tensor_list = {cd.chrom: torch.from_numpy(eng.projected_tensor(i))
+                                      for i, cd in enumerate(schic)}
+        return self
+
+    def transform(self, schic, do_conv, do_rwr, do_col):
+        """Final meta-embedding from the stored projections and factors (:742-834)."""
+        print("start transform")
+        meta_embedding = torch.from_numpy(self.engine.transform())
+        return (None, (self.A_list, self.B_dict.values(), self.D_dict.values(), meta_embedding), self.projection_list)
+
+    def fit_transform(self, schic, size_ratio=0.3, n_iter_max=2000, n_iter_parafac=5, do_conv=True, do_rwr=False,
+                      do_col=False, tol=1e-8, size_list=None, gpu_id=None, verbose=True, run_init=True):
+        print("n_iter_parafac", n_iter_parafac)
+        self.fit(schic, size_ratio, n_iter_max, n_iter_parafac, do_conv, do_rwr, do_col, tol, size_list, gpu_id,
+                 verbose, run_init)
+        return self.transform(schic, do_conv, do_rwr, do_col)

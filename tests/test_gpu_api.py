@@ -1,0 +1,238 @@
+"""Drop-in API on the device: Sparse / Chrom_Dataset / partial_rwr / project2orthogonal /
+parafac / Fast_Higashi_core / FastHigashi, each against the numpy oracle."""
+import json
+import os
+import pickle
+
+import numpy as np
+import pytest
+import torch
+
+from fasthigashi import synthetic as syn
+from tests.helpers import build_dataset, canonical_corr, random_init
+
+pytestmark = pytest.mark.gpu
+
+
+def _wl():
+    return syn.scaled(syn.WORKLOADS["c1"], n_cells=160, bins=[120, 100, 90], rank=32, off_diag=30, band=30,
+                      name="c1_tiny")
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    wl = _wl()
+    data, raw, nis = build_dataset(wl, bs_bin=50)
+    return wl, data, raw, nis
+
+
+def test_sparse_reference_selftest(cuda):
+    from fasthigashi import sparse_for_schic
+    sparse_for_schic.test()
+
+
+def test_sparse_sort_matches_oracle(cuda, tiny):
+    from fasthigashi.sparse_for_schic import Sparse
+    wl, data, raw, _ = tiny
+    for r in raw:
+        s = Sparse(r["idx"], r["val"], r["shape"])
+        s.sort_indices()
+        assert np.array_equal(s.indices, r["sidx"])
+        assert np.array_equal(s.values, r["sval"])
+        assert np.array_equal(s.indptr, r["indptr"])
+
+
+def _datasets(wl, raw, bs_bin=50, bs_cell=None, res=None):
+    from fasthigashi.sparse_for_schic import Chrom_Dataset, Sparse
+    out = []
+    for c, r in enumerate(raw):
+        s = Sparse(r["idx"], r["val"], r["shape"])
+        N = r["shape"][2]
+        out.append(Chrom_Dataset(s, bs_bin=min(bs_bin, r["shape"][0]), bs_cell=bs_cell or N, compact=True,
+                                 flank=wl.off_diag, chrom="chr%d" % (c + 1),
+                                 resolution=res or wl.resolution))
+    return out
+
+
+def test_chrom_dataset_fetch_matches_oracle(cuda, tiny):
+    from oracle import ref_np as R
+    wl, data, raw, _ = tiny
+    ds = _datasets(wl, raw, bs_cell=64)
+    for cd, r in zip(ds, raw):
+        tiles = R.tile_plan(r["shape"][0], cd.bs_bin, wl.off_diag)
+        assert [(t.start, t.rows, t.width, t.col_start, t.local_start, t.shift) for t in tiles] == \
+            [(t.start, t.rows, t.width, t.col_start, t.local_start, t.shift) for t in cd.tiles]
+        chunks = R.cell_slices(cd.num_cell, cd.total_cell_num, cd.bs_cell)
+        assert chunks == cd.cell_slice_list
+        for b, tl in enumerate(tiles):
+            for k, ch in enumerate(chunks):
+                coo, v, ps = R.tile_coo(r["sidx"], r["sval"], r["indptr"], r["shape"], tl, ch)
+                for tr in (True, False):
+                    want = R.densify(coo, v, ps, transpose=tr)
+                    (got, _), kind = cd.fetch(b, k, transpose=tr)
+                    assert kind == "hic"
+                    assert np.array_equal(got.cpu().numpy(), want), (b, k, tr)
+
+
+@pytest.mark.parametrize("do_conv,do_col,epochs", [(False, False, -1), (True, False, 5), (False, True, -1)])
+def test_partial_rwr_matches_oracle(cuda, tiny, do_conv, do_col, epochs):
+    from fasthigashi.partial_rwr import partial_rwr
+    from oracle import ref_np as R
+    wl, data, raw, _ = tiny
+    r = raw[0]
+    tl = R.tile_plan(r["shape"][0], 50, wl.off_diag)[1]
+    coo, v, ps = R.tile_coo(r["sidx"], r["sval"], r["indptr"], r["shape"], tl, slice(0, 96))
+    x = R.densify(coo, v, ps, transpose=True)
+    rng = np.random.default_rng(0)
+    cov = (rng.random((x.shape[0], x.shape[2])) + 0.5).astype(np.float32)
+    want, n_want = R.partial_rwr(x.copy(), tl.local_start, tl.local_start + tl.rows, do_conv, True, do_col,
+                                 bin_cov=cov, force_rwr_epochs=epochs)
+    got, n_got = partial_rwr(torch.from_numpy(x).cuda(), tl.local_start, tl.local_start + tl.rows, do_conv, True,
+                             do_col, bin_cov=torch.from_numpy(cov), return_rwr_iter=True,
+                             force_rwr_epochs=epochs)
+    assert n_got == n_want
+    g = got.cpu().numpy()
+    assert g.shape == want.shape
+    err = np.abs(g - want).max() / np.abs(want).max()
+    assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("shape", [(7, 40, 24), (3, 24, 40), (1, 300, 64)])
+def test_project2orthogonal(cuda, shape):
+    from fasthigashi.project2orthogonal import project2orthogonal
+    from oracle import ref_np as R
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal(shape).astype(np.float32)
+    want, S_want = R.project2orthogonal(X.astype(np.float64))
+    got, S = project2orthogonal(torch.from_numpy(X).cuda())
+    assert got.shape == X.shape and S.shape == S_want.shape
+    assert np.abs(got.cpu().numpy() - want).max() < 1e-5
+    assert np.abs(S.cpu().numpy() - S_want).max() / S_want.max() < 1e-5
+
+
+def test_parafac_matches_oracle(cuda):
+    from fasthigashi.parafac_integrative import parafac
+    from oracle import ref_np as R
+    rng = np.random.default_rng(2)
+    n, s, K = 70, 12, 20
+    A0, B0, D0 = (rng.random((n, s)) + 0.5), (rng.random((s, s)) + 0.5), (rng.random((K, s)) + 0.5)
+    X = np.einsum("ir,jr,kr->ijk", A0, B0, D0) + 0.01 * rng.standard_normal((n, s, K))
+    init = [rng.random((n, s)) + 0.1, rng.random((s, s)) + 0.1, rng.random((K, s)) + 0.1]
+    want, norm_w, x_w, _ = R.parafac(X.astype(np.float64), s, 4, [f.copy() for f in init])
+    got, norm_g, x_g = parafac(torch.from_numpy(X.astype(np.float32)), s, 4,
+                               [torch.from_numpy(f.astype(np.float32)) for f in init])
+    for g, w in zip(got, want):
+        assert np.abs(g.cpu().numpy() - w).max() / np.abs(w).max() < 1e-3
+    assert abs(norm_g - norm_w) / norm_w < 1e-4
+    assert abs(x_g - x_w) / x_w < 1e-4
+
+
+def _inject(model, F0, ds, n_i, bin_cov=None):
+    model.A_list = [torch.from_numpy(a.copy()) for a in F0.A]
+    model.B_dict = {cd.chrom: torch.from_numpy(b.copy()) for cd, b in zip(ds, F0.B)}
+    model.D_dict = {cd.chrom: torch.from_numpy(d.copy()) for cd, d in zip(ds, F0.D)}
+    model.meta_embedding = torch.from_numpy(F0.M.copy())
+    model.n_i = np.array(n_i)
+    model.bin_cov_list = bin_cov
+    model.bad_bin_cov_list = [0] * len(ds)
+
+
+def test_core_fit_transform_matches_oracle(cuda, tiny):
+    from fasthigashi.parafac2_intergrative import Fast_Higashi_core
+    from oracle import ref_np as R
+    wl, _, raw, nis = tiny
+    data, _, _ = build_dataset(wl, bs_bin=50, n_i=nis)          # X at the fixed epochs n_i
+    F0, sizes = random_init(data, wl, wl.rank, seed=5)
+    ds = _datasets(wl, raw)
+    model = Fast_Higashi_core(rank=wl.rank, off_diag=wl.off_diag, res_list=[wl.resolution]).to("cuda")
+    _inject(model, F0, ds, nis)
+    n_it = 5
+    out = model.fit_transform(ds, size_ratio=wl.dim1, n_iter_max=n_it, n_iter_parafac=1, do_conv=False,
+                              do_rwr=True, do_col=False, tol=0, run_init=False, verbose=False)
+    ref = R.fit(data, F0, n_it, 1, 0.0, np.float64)
+    rel = np.abs(np.array(model.rec_errors) - np.array(ref.rec_errors)) / np.array(ref.rec_errors)
+    assert rel.max() < 1e-4, rel
+    assert [model.chrom2size[cd.chrom] for cd in ds] == sizes
+    assert [cd.global_slice_bin for cd in ds] == [slice(0, cd.num_bin) for cd in ds]
+    weights, (A_list, B_vals, D_vals, M), p_list = out
+    assert weights is None and len(p_list) == len(ds)
+    assert [p.shape for p in p_list[0]] == [(t.rows, t.width, sizes[0]) for t in ds[0].tiles]
+    M_ref = R.transform(data, ref.factors, ref.projections, np.float64)
+    cc = canonical_corr(M.numpy(), M_ref)
+    assert cc.min() > 0.999, cc
+    for a, a_ref in zip(model.A_list, ref.factors.A):
+        assert np.abs(a.numpy() - a_ref).max() / np.abs(a_ref).max() < 1e-2
+
+
+def test_core_init_params(cuda, tiny):
+    """init_params on the device: RWR epoch counts and bin coverage match the oracle."""
+    from fasthigashi.parafac2_intergrative import Fast_Higashi_core
+    from oracle import ref_np as R
+    wl, data, raw, nis = tiny
+    ds = _datasets(wl, raw, bs_cell=100)
+    model = Fast_Higashi_core(rank=wl.rank, off_diag=wl.off_diag, res_list=[wl.resolution]).to("cuda")
+    model.svd_random_state = 0
+    model._sizes(ds, wl.dim1, None)
+    model.init_params(ds, do_conv=False, do_rwr=True, do_col=True)
+    N = ds[0].num_cell
+    for c, (cd, r) in enumerate(zip(ds, raw)):
+        cov = np.full((N, cd.num_bin), 1e-4, dtype=np.float64)
+        ni = []
+        for tl in cd.tiles:
+            for ch in cd.cell_slice_list:
+                coo, v, ps = R.tile_coo(r["sidx"], r["sval"], r["indptr"], r["shape"], tl, ch)
+                x = R.densify(coo, v, ps, transpose=True)
+                out, n = R.partial_rwr(x, tl.local_start, tl.local_start + tl.rows, False, True, False)
+                ni.append(n)
+                cov[ch, tl.col_start: tl.col_start + tl.width] += out.sum(1)
+        assert model.n_i[c] == max(ni)
+        cov[cov <= 1e-4] = np.inf
+        got = model.bin_cov_list[c].cpu().numpy()
+        fin = np.isfinite(cov)
+        assert np.array_equal(np.isfinite(got), fin)
+        assert np.abs(got[fin] - cov[fin]).max() / cov[fin].max() < 1e-5
+    M = model.meta_embedding.numpy()
+    assert M.shape == (N, wl.rank)
+    assert np.abs(M.T @ M - np.eye(wl.rank)).max() < 1e-4
+
+
+def test_wrapper_end_to_end(cuda, tmp_path):
+    """FastHigashi on files in the reference's intermediate format: packing bit-exact with the
+    oracle, a short fit, the cell embedding."""
+    import pandas as pd
+    from fasthigashi.FastHigashi_Wrapper import FastHigashi
+    from oracle import ref_np as R
+    wl = syn.scaled(_wl(), n_cells=96, name="wrapper_tiny")
+    tab = syn.generate(wl)
+    temp = tmp_path / "temp"
+    (temp / "raw").mkdir(parents=True)
+    chroms = ["chr%d" % (c + 1) for c in range(len(wl.bins))]
+    mats_all = []
+    for c, chrom in enumerate(chroms):
+        mats = syn.per_cell_csr(tab, c, wl.n_cells)
+        mats_all.append(mats)
+        arr = np.empty(len(mats), dtype=object)
+        arr[:] = mats
+        np.save(temp / "raw" / ("%s_sparse_adj.npy" % chrom), arr, allow_pickle=True)
+    with open(tmp_path / "label_info.pickle", "wb") as f:
+        pickle.dump({"cell_type": list(np.arange(wl.n_cells) % 3)}, f)
+    cfg = dict(chrom_list=chroms, temp_dir=str(temp), data_dir=str(tmp_path), resolution_fh=[wl.resolution],
+               resolution=wl.resolution)
+    with open(tmp_path / "config.JSON", "w") as f:
+        json.dump(cfg, f)
+    fh = FastHigashi(str(tmp_path / "config.JSON"), None, str(tmp_path / "out"), off_diag=wl.off_diag,
+                     filter=False, do_conv=False, do_rwr=True, do_col=False, no_col=False)
+    fh.prep_dataset()
+    for c, cd in enumerate(fh.all_matrix):
+        mats = [mats_all[c][i] for i in fh.reorder]
+        idx, val, shape, _ = R.pack_chrom(mats, wl.off_diag)
+        sidx, sval, indptr = R.sort_indices(idx, val, shape)
+        assert np.array_equal(cd._host.indices, sidx) and np.array_equal(cd._host.values, sval)
+    fh.run_model(dim1=wl.dim1, rank=wl.rank, n_iter_max=3, tol=0)
+    rec = fh.model.rec_errors
+    assert len(rec) == 3 and rec[-1] < rec[0]
+    store = fh.fetch_cell_embedding(final_dim=8)
+    assert store["embed_raw"].shape == (wl.n_cells, 8)
+    assert "embed_correct_coverage_fh" in store
+    fh.load_model(dim1=wl.dim1, rank=wl.rank)
+    assert fh.meta_embedding.shape == (wl.n_cells, wl.rank)

@@ -204,8 +204,10 @@ def test_csr_sort_matches_oracle(cuda):
         oi = torch.zeros_like(di)
         ov = torch.zeros_like(dv)
         ptr = torch.zeros(n_rows + 1, dtype=torch.int64, device=cuda)
+        perm = torch.zeros(nnz, dtype=torch.int64, device=cuda)
         L.call("fh_csr_sort", di.data_ptr(), dv.data_ptr(), nnz, n_rows, chunk, oi.data_ptr(), ov.data_ptr(),
-               ptr.data_ptr(), wsb.data_ptr(), ws, ops.stream_handle())
+               ptr.data_ptr(), perm.data_ptr(), wsb.data_ptr(), ws, ops.stream_handle())
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(idx[0], kind="stable"))
         assert np.array_equal(oi.cpu().numpy(), ridx)
         assert np.array_equal(ov.cpu().numpy(), rval)
         assert np.array_equal(ptr.cpu().numpy(), rptr)
