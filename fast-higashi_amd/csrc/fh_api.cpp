@@ -2,6 +2,8 @@
 // libfh_hip.so returns an int status (0 = OK); fh_last_error() gives the message of
 // the last failure on the calling thread.
 #include <hip/hip_runtime.h>
+
+#include "fh_api.h"
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>

@@ -4,6 +4,8 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#include "fh_api.h"
+
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -11,19 +13,7 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-// dtype codes shared with the C-ABI (fh_api.h)
-enum FhDtype { FH_F32 = 0, FH_BF16 = 1, FH_F64 = 2, FH_FP8 = 3 };
-
 #define FH_WAVE 64
-
-// status codes (0 = OK)
-enum FhStatus {
-  FH_OK = 0,
-  FH_ERR_ARG = -1,
-  FH_ERR_HIP = -2,
-  FH_ERR_UNSUPPORTED = -3,
-  FH_ERR_NUMERIC = -4,
-};
 
 // thread-local last error message (fh_api.cpp)
 void fh_set_error(const char* fmt, ...);

@@ -9,24 +9,6 @@
 // and reduction steps between them.  blockIdx.y selects the chromosome.
 #include "fh_common.h"
 
-extern "C" {
-typedef struct FhCpChrom {
-  float* A;
-  float* B;
-  float* D;
-  const float* T;
-  float* W1;
-  float* MF;
-  float* KR;
-  const float* MDpart;
-  double* G;        // 4 x s x sp: A^T A, B^T B, D^T D, system matrix
-  double* scal;     // [||T||^2, <M_D, D>, ||[[A,B,D]]||^2, spare]
-  double* state;    // [previous loss, have previous]
-  int* flag;        // inner loop stopped
-  double* core;     // output slot for the core norm
-  int64_t n, s, sp, R, Dld, mdsplit;
-} FhCpChrom;
-}
 
 namespace {
 constexpr int NT = 256;

@@ -12,7 +12,6 @@
 // so every MFMA operand fetch is two ds_read_b128.  Operands whose contiguous
 // dimension is M (or N) are transposed in registers on the way into LDS.
 #include "fh_common.h"
-#include "fh_gemm.h"
 
 namespace {
 

@@ -111,20 +111,6 @@ __global__ void gather_coo_kernel(const int32_t* __restrict__ in_idx, const floa
 
 }  // namespace
 
-extern "C" {
-
-typedef struct FhBandTile {
-  int64_t offset;      // element offset of the tile in the output buffer
-  int64_t s_row;       // stride of a local row
-  int64_t s_col;       // stride of a local column
-  int64_t s_cell;      // stride of a cell
-  int32_t row_start;   // first bin of the tile
-  int32_t rows;        // bins in the tile
-  int32_t shift;       // absolute column -> local column offset
-  int32_t width;       // local columns
-} FhBandTile;
-
-}
 
 namespace {
 

@@ -18,20 +18,6 @@ constexpr int NT = 256;
 
 }  // namespace
 
-extern "C" {
-typedef struct FhNsGroup {
-  double* Y0;
-  double* Z0;
-  double* Y1;
-  double* Z1;
-  double* T;
-  double* scale;
-  double* res;
-  int64_t s, ld, stride, count;
-  int64_t tile_begin;
-  int32_t tiles, tm;
-} FhNsGroup;
-}
 
 namespace {
 
@@ -255,14 +241,6 @@ constexpr int CHOL_LDS = 160 * 1024 - 4096;   // bytes available for the packed 
 __device__ __forceinline__ int64_t pk(int64_t i, int64_t j) { return i * (i + 1) / 2 + j; }  // i >= j
 
 }  // namespace
-extern "C" {
-typedef struct FhSpdSys {
-  const double* G;
-  double* out;
-  int64_t s, ldg, ldo;
-  double eps;
-} FhSpdSys;
-}
 namespace {
 
 __global__ __launch_bounds__(NT) void spd_inverse_kernel(const FhSpdSys* __restrict__ sys, int* __restrict__ status) {

@@ -82,66 +82,13 @@ __global__ void krp_kernel(const float* A, int64_t lda, const float* B, int64_t 
 // W1 rows (i*sp + j), columns r (ld = ldw).
 // mode 0:  out[i][r] = sum_j W1[i*sp+j][r] * B[j][r]      (i < n)
 // mode 1:  out[j][r] = sum_i W1[i*sp+j][r] * A[i][r]      (j < s)
-__global__ void mttkrp_reduce_kernel(const float* W1, int64_t ldw, const float* F, int64_t ldf, int64_t n,
-                                     int64_t s, int64_t sp, int mode, float* out, int64_t ldo) {
-  const int64_t rows = mode == 0 ? n : s;
-  const int64_t tot = rows * s;
-  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t)gridDim.x * NT) {
-    int64_t a = e / s, r = e % s;
-    double acc = 0.;
-    if (mode == 0) {
-      for (int64_t j = 0; j < s; ++j) acc += (double)W1[(a * sp + j) * ldw + r] * (double)F[j * ldf + r];
-    } else {
-      for (int64_t i = 0; i < n; ++i) acc += (double)W1[(i * sp + a) * ldw + r] * (double)F[i * ldf + r];
-    }
-    out[a * ldo + r] = (float)acc;
-  }
-}
 
 // out = G1 o G2 (o G3)  s x s fp64
-__global__ void hadamard_kernel(const double* G1, const double* G2, const double* G3, int64_t s, int64_t ld,
-                                double* out) {
-  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < s * s; e += (int64_t)gridDim.x * NT) {
-    int64_t i = e / s, j = e % s;
-    double v = G1[i * ld + j] * G2[i * ld + j];
-    if (G3) v *= G3[i * ld + j];
-    out[i * ld + j] = v;
-  }
-}
 
 // balance_norm (parafac_integrative.py:19-26) on one chromosome, one block per column r:
 // every factor column is unit-normalised, D's column also takes the product of the norms.
-__global__ void balance_kernel(float* A, int64_t lda, int64_t n, float* B, int64_t ldb, float* D, int64_t ldd,
-                               int64_t R, int64_t s, const int* skip) {
-  if (skip && *skip) return;
-  __shared__ double red[NT / 64];
-  const int64_t r = blockIdx.x;
-  double pa = 0., pb = 0., pd = 0.;
-  for (int64_t i = threadIdx.x; i < n; i += NT) { double v = A[i * lda + r]; pa += v * v; }
-  for (int64_t i = threadIdx.x; i < s; i += NT) { double v = B[i * ldb + r]; pb += v * v; }
-  for (int64_t i = threadIdx.x; i < R; i += NT) { double v = D[i * ldd + r]; pd += v * v; }
-  const float na = (float)sqrt(block_sum(pa, red));
-  const float nb = (float)sqrt(block_sum(pb, red));
-  const float nd = (float)sqrt(block_sum(pd, red));
-  const float even = na * nb * nd + 1e-15f;
-  const float ia = 1.f / (na + 1e-15f), ib = 1.f / (nb + 1e-15f), id = even / (nd + 1e-15f);
-  for (int64_t i = threadIdx.x; i < n; i += NT) A[i * lda + r] *= ia;
-  for (int64_t i = threadIdx.x; i < s; i += NT) B[i * ldb + r] *= ib;
-  for (int64_t i = threadIdx.x; i < R; i += NT) D[i * ldd + r] *= id;
-}
 
 // out[0] = sum_{ij} G1 o G2 o G3  (= ||[[A,B,D]]||^2 from the three Grams)
-__global__ void sum3_kernel(const double* G1, const double* G2, const double* G3, int64_t s, int64_t ld,
-                            double* out) {
-  __shared__ double red[NT / 64];
-  double p = 0.;
-  for (int64_t e = threadIdx.x; e < s * s; e += NT) {
-    int64_t i = e / s, j = e % s;
-    p += G1[i * ld + j] * G2[i * ld + j] * G3[i * ld + j];
-  }
-  double t = block_sum(p, red);
-  if (threadIdx.x == 0) *out = t;
-}
 
 // partial sums of squares over rows x cols (ld) blocks: part[blockIdx.x]
 __global__ void sumsq_partial_kernel(const void* X, int dt, int64_t rows, int64_t cols, int64_t ld,
@@ -172,17 +119,6 @@ __global__ void segment_sum_kernel(const double* vals, const int64_t* offsets, i
 }
 
 // <X, Y> over rows x cols with independent strides (fp32/fp64 inputs), written to out[0]
-__global__ void dot2d_kernel(const void* X, int xdt, int64_t ldx, const void* Y, int ydt, int64_t ldy,
-                             int64_t rows, int64_t cols, double* out) {
-  __shared__ double red[NT / 64];
-  double p = 0.;
-  for (int64_t e = threadIdx.x; e < rows * cols; e += NT) {
-    int64_t r = e / cols, c = e % cols;
-    p += load_any(X, xdt, r * ldx + c) * load_any(Y, ydt, r * ldy + c);
-  }
-  double t = block_sum(p, red);
-  if (threadIdx.x == 0) *out = t;
-}
 
 // Sweep error bookkeeping (parafac2_intergrative.py:667-668,710):
 //   err_U[c] = tnorm[c] + core[c] - 2 xU[c],  err_V = sum tnorm + sum core - 2 xV
@@ -226,24 +162,10 @@ __global__ void diag_kernel(float* X, int64_t nb, int64_t stride, int64_t ld, in
 // CP-ALS inner-iteration test (parafac_integrative.py:80-108) from closed forms:
 //   loss = ||T||^2 - 2 <M_D, D> + ||[[A,B,D]]||^2 ; wdiff = (prev - loss) / prev; stop if wdiff < 1e-5
 // state: [0] = previous loss, [1] = have previous; flag: 1 once stopped.
-__global__ void cpals_check_kernel(const double* tnorm2, const double* inner, const double* xhat2, double* state,
-                                   int* flag) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (*flag) return;
-  double loss = tnorm2[0] - 2. * inner[0] + xhat2[0];
-  if (state[1] != 0.) {
-    double wdiff = (state[0] - loss) / state[0];
-    if (wdiff < 1e-5) *flag = 1;
-  }
-  state[0] = loss;
-  state[1] = 1.;
-}
 
 }  // namespace
 
 extern "C" {
-
-typedef struct FhRowScale { float* Y; const float* A; int64_t nrows, w, s, ldy, lda; } FhRowScale;
 
 static inline unsigned grid_for(int64_t n) {
   int64_t g = (n + NT - 1) / NT;
@@ -285,36 +207,6 @@ int fh_krp(const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n, 
   return FH_OK;
 }
 
-int fh_mttkrp_reduce(const float* W1, int64_t ldw, const float* F, int64_t ldf, int64_t n, int64_t s,
-                     int64_t sp, int mode, float* out, int64_t ldo, hipStream_t st) {
-  int64_t tot = (mode == 0 ? n : s) * s;
-  hipLaunchKernelGGL(mttkrp_reduce_kernel, dim3(grid_for(tot)), dim3(NT), 0, st, W1, ldw, F, ldf, n, s, sp, mode,
-                     out, ldo);
-  FH_CHECK_LAUNCH();
-  return FH_OK;
-}
-
-int fh_hadamard(const double* G1, const double* G2, const double* G3, int64_t s, int64_t ld, double* out,
-                hipStream_t st) {
-  hipLaunchKernelGGL(hadamard_kernel, dim3(grid_for(s * s)), dim3(NT), 0, st, G1, G2, G3, s, ld, out);
-  FH_CHECK_LAUNCH();
-  return FH_OK;
-}
-
-int fh_balance_norm(float* A, int64_t lda, int64_t n, float* B, int64_t ldb, float* D, int64_t ldd, int64_t R,
-                    int64_t s, const int* skip, hipStream_t st) {
-  hipLaunchKernelGGL(balance_kernel, dim3((unsigned)s), dim3(NT), 0, st, A, lda, n, B, ldb, D, ldd, R, s, skip);
-  FH_CHECK_LAUNCH();
-  return FH_OK;
-}
-
-int fh_sum3(const double* G1, const double* G2, const double* G3, int64_t s, int64_t ld, double* out,
-            hipStream_t st) {
-  hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(NT), 0, st, G1, G2, G3, s, ld, out);
-  FH_CHECK_LAUNCH();
-  return FH_OK;
-}
-
 // partial sums of squares; part must hold ceil(rows / rows_per_block) doubles
 int fh_sumsq_partial(const void* X, int dt, int64_t rows, int64_t cols, int64_t ld, int64_t rows_per_block,
                      double* part, hipStream_t st) {
@@ -331,13 +223,6 @@ int fh_segment_sum(const double* vals, const int64_t* offsets_dev, int n_seg, do
                    hipStream_t st) {
   if (n_seg <= 0) return FH_OK;
   hipLaunchKernelGGL(segment_sum_kernel, dim3(n_seg), dim3(NT), 0, st, vals, offsets_dev, n_seg, out, accumulate);
-  FH_CHECK_LAUNCH();
-  return FH_OK;
-}
-
-int fh_dot2d(const void* X, int xdt, int64_t ldx, const void* Y, int ydt, int64_t ldy, int64_t rows,
-             int64_t cols, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(dot2d_kernel, dim3(1), dim3(NT), 0, st, X, xdt, ldx, Y, ydt, ldy, rows, cols, out);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }
@@ -364,13 +249,6 @@ int fh_batch_sum(const void* in, int dt, int64_t nb, int64_t stride, int64_t cou
 int fh_diag(float* X, int64_t nb, int64_t stride, int64_t ld, int64_t n, float v, int add, hipStream_t st) {
   if (nb * n <= 0) return FH_OK;
   hipLaunchKernelGGL(diag_kernel, dim3(grid_for(nb * n)), dim3(NT), 0, st, X, nb, stride, ld, n, v, add);
-  FH_CHECK_LAUNCH();
-  return FH_OK;
-}
-
-int fh_cpals_check(const double* tnorm2, const double* inner, const double* xhat2, double* state, int* flag,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(cpals_check_kernel, dim3(1), dim3(64), 0, st, tnorm2, inner, xhat2, state, flag);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }

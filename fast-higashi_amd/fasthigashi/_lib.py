@@ -64,12 +64,6 @@ class CpChrom(Structure):
                 ("mdsplit", c_int64)]
 
 
-class LossTile(Structure):
-    _fields_ = [("X", c_void_p), ("P", c_void_p), ("Y", c_void_p), ("gY", c_void_p), ("gP", c_void_p),
-                ("w", c_int64), ("s", c_int64), ("n", c_int64), ("ldx", c_int64), ("ldp", c_int64),
-                ("ldy", c_int64), ("rows", c_int64), ("out_slot", c_int64)]
-
-
 # name -> (restype, argtypes)
 _SIGS = {
     "fh_last_error": (c_char_p, []),
@@ -92,20 +86,11 @@ _SIGS = {
     "fh_scale_rows": (c_int, [c_void_p, c_int, c_int64, c_void_p]),
     "fh_krp": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int,
                        c_void_p]),
-    "fh_mttkrp_reduce": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int,
-                                 c_void_p, c_int64, c_void_p]),
-    "fh_hadamard": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
-    "fh_balance_norm": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_int64,
-                                c_int64, c_void_p, c_void_p]),
-    "fh_sum3": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
     "fh_sumsq_partial": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "fh_segment_sum": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
-    "fh_dot2d": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p,
-                         c_void_p]),
     "fh_sweep_errors": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "fh_batch_sum": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "fh_diag": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_float, c_int, c_void_p]),
-    "fh_cpals_check": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fh_cpals_op": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_void_p]),
     "fh_cpals_tnorm": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "fh_csr_sort_workspace": (c_int64, [c_int64, c_int32, c_int64]),

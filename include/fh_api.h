@@ -1,0 +1,251 @@
+/*
+ * fh_api.h — C ABI of libfh_hip.so, the gfx950 (MI355X) kernels of the Fast-Higashi
+ * decomposition path.
+ *
+ * Conventions (every entry point):
+ *  - pointers are device pointers unless a comment says "host";
+ *  - sizes, leading dimensions and strides are int64_t element counts;
+ *  - work is enqueued on `stream` (a hipStream_t; NULL = the legacy default stream) and
+ *    never synchronises the host, except fh_stream_sync and the two host-side table
+ *    helpers fh_gemm_prepare / fh_ns_prepare (which only touch host memory);
+ *  - the return value is 0 (FH_OK) or a negative FhStatus; fh_last_error() then returns
+ *    a thread-local message;
+ *  - the caller owns every buffer; the library allocates nothing.
+ *
+ * Each declaration names the reference code it replaces (file:line in
+ * ma-compbio/Fast-Higashi, fasthigashi/).  Entry points marked "no counterpart" are
+ * plumbing of this library.
+ */
+#ifndef FH_API_H
+#define FH_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t is `struct ihipStream_t*`; declared here so C callers need no HIP headers. */
+struct ihipStream_t;
+typedef struct ihipStream_t* fh_stream_t;
+
+/* ------------------------------------------------------------------ codes */
+enum FhDtype { FH_F32 = 0, FH_BF16 = 1, FH_F64 = 2, FH_FP8 = 3 };
+enum FhStatus {
+  FH_OK = 0,
+  FH_ERR_ARG = -1,         /* argument outside the documented range */
+  FH_ERR_HIP = -2,         /* a HIP runtime call or launch failed */
+  FH_ERR_UNSUPPORTED = -3, /* dtype / layout combination without a kernel */
+  FH_ERR_NUMERIC = -4,
+};
+
+/* ------------------------------------------------------------------ plumbing (no counterpart) */
+const char* fh_last_error(void);
+void fh_clear_error(void);
+int fh_version(void);                     /* 100 = 0.1.0 */
+const char* fh_target(void);              /* "gfx950" */
+int fh_stream_sync(fh_stream_t stream);   /* hipStreamSynchronize with status mapping */
+
+/* ------------------------------------------------------------------ grouped GEMM
+ * C[b] (M x N) = alpha * A[b] (M x K) * B[b] (K x N) + beta * C[b] (+ diag on the diagonal).
+ * Element (m, k) of A is at A + b*sAb + m*sAm + k*sAk (likewise B, C).  For the fp32 /
+ * bf16 / fp8 kernels exactly one of sAm, sAk is 1 (and of sBk, sBn), the leading dimension
+ * and batch strides are multiples of 16 bytes / element size, and pointers are 16-byte
+ * aligned.
+ *
+ * Replaces every batched product of the sweep and of the imputation:
+ *   parafac2_intergrative.py:336 (C = M D), :374-383 (lhs contraction + bmm into temp),
+ *   :422-430 and :470-471 (P^T X and the SVD_term product), :522-526 (pass-3 contraction),
+ *   parafac_integrative.py:43-62 (Grams and MTTKRP of CP-ALS),
+ *   partial_rwr.py:85 (A A^T), :117 (Q P), :138 (Q A).
+ */
+typedef struct FhGemmDesc {
+  const void* A;
+  const void* B;
+  void* C;
+  int64_t M, N, K;
+  int64_t sAm, sAk, sBk, sBn, sCm, sCn;
+  int64_t sAb, sBb, sCb;
+  int64_t sCk;            /* FH_GEMM_SPLIT_PARTIAL: stride between the k-split partial outputs */
+  int32_t batch;
+  int32_t flags;          /* FH_GEMM_* | (k-split count << 8) */
+  double alpha, beta, diag;
+  const float* scale_a;   /* fp8: per-batch dequant scale of A (nullable) */
+  const float* scale_b;   /* fp8: per-batch dequant scale of B (nullable) */
+  const int* skip;        /* nullable device flag: when *skip != 0 the descriptor is a no-op */
+  int64_t tile_begin;     /* written by fh_gemm_prepare */
+  int32_t tiles_m, tiles_n;
+} FhGemmDesc;
+
+enum {
+  FH_GEMM_ATOMIC = 1,        /* C += alpha*AB with atomics (beta ignored) */
+  FH_GEMM_OUT_BF16 = 2,      /* C stored as bf16 */
+  FH_GEMM_SPLIT_PARTIAL = 4, /* k-split ks writes C + ks*sCk (fixed-order reduction later) */
+};
+
+/* host: fill tile_begin/tiles_m/tiles_n of n descriptors for a bm x bn tile; returns total tiles */
+int64_t fh_gemm_prepare(FhGemmDesc* descs_host, int n, int bm, int bn);
+/* host: tile dims of shape code 0..6 (128x128, 128x64, 64x64, 128x160, 64x128, 256x128, 128x256) */
+void fh_gemm_shape_dims(int shape, int* bm, int* bn);
+/* dtype: FH_F32 / FH_BF16 / FH_FP8 operands, fp32 accumulate (MFMA 16x16x4 f32, 16x16x32 bf16,
+ * 16x16x32 fp8).  ak / bk: 1 if A (B) is k-contiguous, 0 if m- (n-) contiguous.  descs_dev: the
+ * prepared table (n_desc entries) or NULL to launch *single_host. */
+int fh_gemm_launch(int dtype, int shape, int ak, int bk, const FhGemmDesc* descs_dev, int n_desc,
+                   const FhGemmDesc* single_host, int64_t total_tiles, fh_stream_t stream);
+/* FP64 MFMA (16x16x4 f64) GEMM; operands fp32 or fp64 (widened on load), C fp32 or fp64.
+ * Replaces the FP32 Gram/solve inputs of project2orthogonal.py:6-31 and
+ * parafac_integrative.py:12-15 with FP64 ones. */
+int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhGemmDesc* descs_dev, int n_desc,
+                     const FhGemmDesc* single_host, int64_t total_tiles, fh_stream_t stream);
+
+/* ------------------------------------------------------------------ polar factor (FP64)
+ * P = Y G^{-1/2} with G = Y^T Y, and the nuclear norm tr(G^{1/2}), by the coupled
+ * Newton-Schulz iteration on G / ||G||_F.  Replaces project2orthogonal.py:6-31
+ * (torch.linalg.svd U Vh and S.sum()) at parafac2_intergrative.py:396-417 (per bin tile)
+ * and :483-486 (meta-embedding).  Each group holds `count` s x s matrices. */
+typedef struct FhNsGroup {
+  double* Y0;   /* in: G (count x s x ld, stride `stride`); out of fh_ns_fused: G^{-1/2} */
+  double* Z0;
+  double* Y1;
+  double* Z1;
+  double* T;
+  double* scale;  /* count doubles: ||G||_F */
+  double* res;    /* count doubles: final residual */
+  int64_t s, ld, stride, count;
+  int64_t tile_begin;  /* written by fh_ns_prepare */
+  int32_t tiles, tm;
+} FhNsGroup;
+
+/* host: fill tile bookkeeping of the multi-launch variant; returns total tiles */
+int64_t fh_ns_prepare(FhNsGroup* groups_host, int n);
+/* multi-launch iteration (one launch per step, any s); *result_buffer_host = 0/1 */
+int fh_ns_run(const FhNsGroup* groups_dev, int n_groups, const int64_t* mat_begin_dev, int64_t total_tiles,
+              int64_t total_mats, int n_iter, double tol, double* nuc_dev, int* result_buffer_host,
+              fh_stream_t stream);
+/* one workgroup per matrix (s <= 160 in LDS-free register tiles, larger s tiled): init,
+ * iterations until ||I - Z Y||_F^2 < tol or max_iter, G^{-1/2} into Y0, nuclear norm into
+ * nuc_dev[mat], iteration count into iters_dev[mat] */
+int fh_ns_fused(const FhNsGroup* groups_dev, int n_groups, const int64_t* mat_begin_dev, int64_t total_mats,
+                int max_iter, double tol, double* nuc_dev, int* iters_dev, fh_stream_t stream);
+
+/* ------------------------------------------------------------------ SPD inverse (FP64)
+ * out = (G + eps I)^{-1} by Cholesky in LDS, one workgroup per system (s <= 256).
+ * Replaces torch.linalg.solve in update_factor, parafac_integrative.py:12-15. */
+typedef struct FhSpdSys {
+  const double* G;
+  double* out;
+  int64_t s, ldg, ldo;
+  double eps;
+} FhSpdSys;
+int fh_spd_inverse(const FhSpdSys* sys_dev, int n, int64_t max_s, int* status_dev, fh_stream_t stream);
+
+/* ------------------------------------------------------------------ CP-ALS (grouped over chromosomes)
+ * parafac_integrative.py:28-112 on every T_c at once: Hadamard of Grams (calc_A :43-48),
+ * MTTKRP reductions (calc_B :50-62), Khatri-Rao rows, loss / wdiff stop (:82-107),
+ * balance_norm (:19-26, :65, :108) and the closed-form core norm
+ * (parafac2_intergrative.py:621-632, 697-706). */
+typedef struct FhCpChrom {
+  float* A;
+  float* B;
+  float* D;
+  const float* T;
+  float* W1;
+  float* MF;
+  float* KR;
+  const float* MDpart;
+  double* G;        /* 4 x s x sp: A^T A, B^T B, D^T D, system matrix */
+  double* scal;     /* [||T||^2, <M_D, D>, ||[[A,B,D]]||^2, spare] */
+  double* state;    /* [previous loss, have previous] */
+  int* flag;        /* inner loop stopped */
+  double* core;     /* output slot for the core norm */
+  int64_t n, s, sp, R, Dld, mdsplit;
+} FhCpChrom;
+enum { FH_CP_HADAMARD = 0, FH_CP_MTTKRP = 1, FH_CP_KRP = 2, FH_CP_CHECK = 3, FH_CP_BALANCE = 4,
+       FH_CP_CORE = 5 };
+int fh_cpals_op(const FhCpChrom* tab_dev, int K, int op, int mode, int64_t max_elems, fh_stream_t stream);
+/* ||T_c||^2 into scal[0]; resets state/flag (start of parafac, parafac_integrative.py:28-66) */
+int fh_cpals_tnorm(const FhCpChrom* tab_dev, int K, double* part, int nblk, fh_stream_t stream);
+
+/* ------------------------------------------------------------------ packing
+ * Stable CSR order by row (idx rows: row, col, cell; nnz entries), n_rows <= 16384.
+ * Replaces Sparse.sort_indices / scipy coo_tocsr, sparse_for_schic.py:133-182. */
+int64_t fh_csr_sort_workspace(int64_t nnz, int32_t n_rows, int64_t chunk);
+int fh_csr_sort(const int32_t* idx, const float* val, int64_t nnz, int32_t n_rows, int64_t chunk,
+                int32_t* out_idx, float* out_val, int64_t* indptr, int64_t* perm_out /* nullable */,
+                void* workspace, int64_t ws_bytes, fh_stream_t stream);
+
+/* one band tile's entries scattered into a dense tile (cells [cell_lo, cell_hi)); replaces
+ * Fake_Sparse + densify_jit + fetch, sparse_for_schic.py:279-353, 588-613 */
+typedef struct FhBandTile {
+  int64_t offset;      /* element offset of the tile in the output buffer */
+  int64_t s_row;       /* stride of a local row */
+  int64_t s_col;       /* stride of a local column */
+  int64_t s_cell;      /* stride of a cell */
+  int32_t row_start;   /* first bin of the tile */
+  int32_t rows;        /* bins in the tile */
+  int32_t shift;       /* absolute column -> local column offset */
+  int32_t width;       /* local columns */
+} FhBandTile;
+int fh_band_scatter(const int32_t* idx, const float* val, int64_t nnz, const int32_t* tile_of_row,
+                    const FhBandTile* tiles_dev, int32_t cell_lo, int32_t cell_hi, void* out, int out_dtype,
+                    unsigned long long* dropped_dev /* nullable */, fh_stream_t stream);
+/* (cells, positions) fp32 -> (positions, cells) in dst_dtype with a position map (-1 = skip):
+ * the (bs, w, N_k) orientation of fetch(transpose=False), sparse_for_schic.py:588-613 */
+int fh_transpose(const float* src, int64_t ld_src, int64_t ncells, int64_t npos, void* dst, int dst_dtype,
+                 int64_t ldd, const int64_t* pos_map, fh_stream_t stream);
+
+/* ------------------------------------------------------------------ partial RWR (partial_rwr.py:45-175)
+ * Tiles are (ncell, rows, ld) fp32 with cell stride `str`. */
+/* transition matrix from S2 = X X^T and the diagonal block of X (:85-111) */
+int fh_rwr_prep(const float* S2, int64_t ldS, int64_t strS, const float* X, int64_t ldX, int64_t strX, int64_t ls,
+                int64_t rows, float* P, int64_t ldP, int64_t strP, int64_t ncell, fh_stream_t stream);
+/* ||Qa - Qb||_F per cell and their max: the auto-stop test (:119-124) */
+int fh_rwr_delta(const float* Qa, const float* Qb, int64_t ld, int64_t str, int64_t rows, int64_t ncell,
+                 float* delta, float* out_max, fh_stream_t stream);
+/* Qo = rownorm(max(0, (Q + Q^T) / 2)) (:131-134) */
+int fh_rwr_symnorm(const float* Q, float* Qo, int64_t ld, int64_t str, int64_t rows, int64_t ncell,
+                   fh_stream_t stream);
+/* X[c, :, k] /= cov[c, k] (:135) */
+int fh_div_cols(float* X, int64_t ldX, int64_t strX, int64_t rows, int64_t w, const float* cov, int64_t ldc,
+                int64_t ncell, fh_stream_t stream);
+/* 3x3 average pool (padding 1, count_include_pad) then clamp 1e-8 (:77-81) */
+int fh_avgpool3(const float* X, int64_t ldX, int64_t strX, int64_t rows, int64_t w, int64_t ncell, float* out,
+                int64_t ldO, int64_t strO, fh_stream_t stream);
+/* init features: ll x ll average pool, stride ll, floor (parafac2_intergrative.py:142-146) */
+int fh_pool(const float* X, int64_t ldX, int64_t strX, int64_t rows, int64_t w, int64_t ll, int64_t ncell,
+            float* F, int64_t ldf, fh_stream_t stream);
+/* bin coverage: cov[c, k] += sum_i X[c, i, k] (parafac2_intergrative.py:136-137) */
+int fh_colsum_add(const float* X, int64_t ldX, int64_t strX, int64_t rows, int64_t w, int64_t ncell, float* cov,
+                  int64_t ldc, fh_stream_t stream);
+
+/* ------------------------------------------------------------------ sweep element-wise / reductions */
+/* fill n elements of esize bytes with the low bytes of pattern (torch.full / ones) */
+int fh_fill(void* p, int esize, int64_t n, uint64_t pattern, fh_stream_t stream);
+/* dst[r*dld + c] = alpha * src[r*srs + c*scs] with dtype conversion (fp32/bf16/fp64) */
+int fh_copy2d(const void* src, int sdt, int64_t srs, int64_t scs, void* dst, int ddt, int64_t dld, int64_t rows,
+              int64_t cols, double alpha, fh_stream_t stream);
+/* temp rows scaled by A rows: the diag(A_i) of lhs, parafac2_intergrative.py:374 */
+typedef struct FhRowScale { float* Y; const float* A; int64_t nrows, w, s, ldy, lda; } FhRowScale;
+int fh_scale_rows(const FhRowScale* tab_dev, int n_tiles, int64_t max_elems, fh_stream_t stream);
+/* Khatri-Rao rows L[i*s + j, :] = A[i, :] * B[j, :]: lhs2 of parafac2_intergrative.py:422-423 */
+int fh_krp(const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n, int64_t s, int64_t sp, void* L,
+           int l_dtype, fh_stream_t stream);
+/* per-block partial sums of squares: ||X||^2 of parafac2_intergrative.py:368-369 */
+int fh_sumsq_partial(const void* X, int dt, int64_t rows, int64_t cols, int64_t ld, int64_t rows_per_block,
+                     double* part, fh_stream_t stream);
+/* out[g] (+)= sum of vals[offsets[g] .. offsets[g+1]): x_U per chromosome, :409-417 */
+int fh_segment_sum(const double* vals, const int64_t* offsets_dev, int n_seg, double* out, int accumulate,
+                   fh_stream_t stream);
+/* err_U, err_V and rec_error of one iteration (parafac2_intergrative.py:667-668, 710) into hist */
+int fh_sweep_errors(const double* tnorm, const double* core, const double* xU, const double* xV, int K,
+                    double* hist, fh_stream_t stream);
+/* out[i] = sum_b in[b*stride + i] in fixed order (split-K reductions, e.g. SVD_term :430) */
+int fh_batch_sum(const void* in, int dt, int64_t nb, int64_t stride, int64_t count, void* out,
+                 fh_stream_t stream);
+/* X[b][i][i] = v (add = 0) or += v (add = 1): torch.eye / diagonal add, partial_rwr.py:113, 118 */
+int fh_diag(float* X, int64_t nb, int64_t stride, int64_t ld, int64_t n, float v, int add, fh_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FH_API_H */

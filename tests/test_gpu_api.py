@@ -121,8 +121,12 @@ def test_parafac_matches_oracle(cuda):
     want, norm_w, x_w, _ = R.parafac(X.astype(np.float64), s, 4, [f.copy() for f in init])
     got, norm_g, x_g = parafac(torch.from_numpy(X.astype(np.float32)), s, 4,
                                [torch.from_numpy(f.astype(np.float32)) for f in init])
+    # the reconstruction is well conditioned; single factor columns are not (FP32 vs FP64)
+    rec_g = np.einsum("ir,jr,kr->ijk", *[g.cpu().numpy().astype(np.float64) for g in got])
+    rec_w = np.einsum("ir,jr,kr->ijk", *want)
+    assert np.abs(rec_g - rec_w).max() / np.abs(rec_w).max() < 1e-4
     for g, w in zip(got, want):
-        assert np.abs(g.cpu().numpy() - w).max() / np.abs(w).max() < 1e-3
+        assert np.abs(g.cpu().numpy() - w).max() / np.abs(w).max() < 1e-2
     assert abs(norm_g - norm_w) / norm_w < 1e-4
     assert abs(x_g - x_w) / x_w < 1e-4
 
@@ -160,8 +164,12 @@ def test_core_fit_transform_matches_oracle(cuda, tiny):
     M_ref = R.transform(data, ref.factors, ref.projections, np.float64)
     cc = canonical_corr(M.numpy(), M_ref)
     assert cc.min() > 0.999, cc
-    for a, a_ref in zip(model.A_list, ref.factors.A):
-        assert np.abs(a.numpy() - a_ref).max() / np.abs(a_ref).max() < 1e-2
+    # the CP reconstruction of each chromosome (single factor columns are ill conditioned)
+    for c, cd in enumerate(ds):
+        got = np.einsum("ir,jr,kr->ijk", model.A_list[c].numpy(), model.B_dict[cd.chrom].numpy(),
+                        model.D_dict[cd.chrom].numpy())
+        want = np.einsum("ir,jr,kr->ijk", ref.factors.A[c], ref.factors.B[c], ref.factors.D[c])
+        assert np.abs(got - want).max() / np.abs(want).max() < 5e-3
 
 
 def test_core_init_params(cuda, tiny):
@@ -182,7 +190,8 @@ def test_core_init_params(cuda, tiny):
             for ch in cd.cell_slice_list:
                 coo, v, ps = R.tile_coo(r["sidx"], r["sval"], r["indptr"], r["shape"], tl, ch)
                 x = R.densify(coo, v, ps, transpose=True)
-                out, n = R.partial_rwr(x, tl.local_start, tl.local_start + tl.rows, False, True, False)
+                out, n = R.partial_rwr(x, tl.local_start, tl.local_start + tl.rows, False, True, False,
+                                        final_transpose=False)
                 ni.append(n)
                 cov[ch, tl.col_start: tl.col_start + tl.width] += out.sum(1)
         assert model.n_i[c] == max(ni)

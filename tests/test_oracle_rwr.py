@@ -1,0 +1,140 @@
+"""Pins the oracle's partial RWR (partial_rwr.py:45-175) by closed forms (CPU, float64).
+
+* tiles with a known transition matrix P (identity block, all-ones, rank-one row-scaled);
+* the power iteration Q_n against the closed form (rP)^n + (1-r) sum_{t<n} (rP)^t with r = 0.5;
+* the auto-stop epoch count against the first n with ||Q_n - Q_{n+1}||_F < 0.01;
+* zero epochs = identity; a row bin without contacts gets a unit diagonal;
+* do_col: the symmetrised, row-normalised Q and the division by the bin coverage."""
+import numpy as np
+import pytest
+
+from oracle import ref_np as R
+
+
+def _q_closed(P, n, r=0.5):
+    m = P.shape[0]
+    rp = r * P
+    acc = np.zeros_like(P)
+    pw = np.eye(m)
+    for _ in range(n):
+        acc += pw
+        pw = pw @ rp
+    return pw + (1 - r) * acc
+
+
+def _tile_rowscaled(d, w, ls):
+    """x[i, :] = d_i on every column: rows scaled, rank one."""
+    rows = len(d)
+    return np.tile(np.asarray(d, np.float64)[:, None], (1, w))[None], ls, rows
+
+
+def _p_rowscaled(d):
+    """P of the rank-one tile derived by hand: the first-order part (0.75) column-normalises the
+    diagonal block (every column of it is d, so d / sum d); the second-order part (0.25)
+    column-normalises A A^T = w d d^T with a zero diagonal (d_i / (S - d_j), i != j)."""
+    d = np.asarray(d, np.float64)
+    S = d.sum()
+    n = len(d)
+    P = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            P[i, j] = 0.75 * d[i] / S + (0.25 * d[i] / (S - d[j]) if i != j else 0.0)
+    return P
+
+
+@pytest.mark.parametrize("epochs", [0, 1, 2, 5, 9])
+def test_rwr_rowscaled_closed_form(epochs):
+    d = [1.0, 2.0, 0.5, 3.0]
+    x, ls, rows = _tile_rowscaled(d, 9, 2)
+    P = _p_rowscaled(d)
+    assert np.allclose(P.sum(0), 1.0)
+    out, n = R.partial_rwr(x.copy(), ls, ls + rows, False, True, False, force_rwr_epochs=epochs,
+                           final_transpose=False)
+    assert n == epochs
+    want = _q_closed(P, epochs) @ x[0]
+    assert np.allclose(out[0], want, rtol=1e-12, atol=1e-14)
+
+
+def test_rwr_zero_epochs_is_identity():
+    x = np.random.default_rng(0).random((3, 5, 11))
+    out, n = R.partial_rwr(x.copy(), 3, 8, False, True, False, force_rwr_epochs=0, final_transpose=False)
+    assert n == 0 and np.array_equal(out, x)
+
+
+def test_rwr_identity_block():
+    """x = [0 | I | 0]: A A^T = I has a zero off-diagonal, so P = I and every Q_n = I; auto-stop ends at 0."""
+    rows, w, ls = 6, 14, 4
+    x = np.zeros((1, rows, w))
+    x[0, :, ls:ls + rows] = np.eye(rows)
+    out, n = R.partial_rwr(x.copy(), ls, ls + rows, False, True, False, final_transpose=False)
+    assert n == 0
+    assert np.allclose(out, x)
+
+
+def test_rwr_all_ones_auto_stop():
+    """x = ones: P = a 11^T + b I with a = 0.75/n + 0.25/(n-1), b = -0.25/(n-1); the epoch count is
+    the first n with ||Q_n - Q_{n+1}||_F < 0.01 and the output is Q_n x."""
+    rows, w = 7, 12
+    x = np.ones((2, rows, w))
+    a, b = 0.75 / rows + 0.25 / (rows - 1), -0.25 / (rows - 1)
+    P = a * np.ones((rows, rows)) + b * np.eye(rows)
+    k = 0
+    while np.linalg.norm(_q_closed(P, k) - _q_closed(P, k + 1)) >= 0.01:
+        k += 1
+    out, n = R.partial_rwr(x.copy(), 2, 2 + rows, False, True, False, final_transpose=False)
+    assert n == k
+    # the loop has already stepped to Q_{k+1} when it stops (partial_rwr.py:119-124)
+    assert np.allclose(out[0], _q_closed(P, k + 1) @ x[0])
+
+
+def test_rwr_zero_column_fill():
+    """Bin 2 of an identity block has no contact at all: its local_sim column is zero and the
+    diagonal fill (partial_rwr.py:96-97) makes P = I again, so Q_1 = Q_0 and auto-stop ends at 0
+    epochs.  Without the fill P e_2 = 0, ||Q_0 - Q_1||_F = 0.5 and the walk would run on."""
+    rows, w, ls = 6, 14, 4
+    x = np.zeros((1, rows, w))
+    x[0, :, ls:ls + rows] = np.diag([1.0, 1.0, 0.0, 1.0, 1.0, 1.0])
+    out, n = R.partial_rwr(x.copy(), ls, ls + rows, False, True, False, final_transpose=False)
+    assert n == 0
+    assert np.allclose(out, x)
+
+
+def test_rwr_limit():
+    """After many epochs Q is the limit (1-r)(I - rP)^{-1} of the row-scaled tile's walk."""
+    d = [1.0, 2.0, 0.5, 3.0, 1.5]
+    x, ls, rows = _tile_rowscaled(d, 11, 3)
+    out, n = R.partial_rwr(x.copy(), ls, ls + rows, False, True, False, force_rwr_epochs=200,
+                           final_transpose=False)
+    P = _p_rowscaled(d)
+    Qinf = 0.5 * np.linalg.inv(np.eye(rows) - 0.5 * P)
+    assert np.allclose(out[0], Qinf @ x[0], rtol=1e-10)
+    assert np.allclose(Qinf.sum(0), 1.0)
+
+
+def test_rwr_do_col():
+    rng = np.random.default_rng(5)
+    rows, w, ls = 5, 9, 2
+    x = rng.random((2, rows, w)) + 0.1
+    cov = rng.random((2, w)) + 0.5
+    base, _ = R.partial_rwr(x.copy(), ls, ls + rows, False, True, False, force_rwr_epochs=4,
+                            final_transpose=False)
+    out, _ = R.partial_rwr(x.copy(), ls, ls + rows, False, True, True, bin_cov=cov, force_rwr_epochs=4,
+                           final_transpose=False)
+    for c in range(2):
+        Q = base[c] @ np.linalg.pinv(x[c])                 # Q_4 of the plain walk
+        Qs = np.maximum((Q + Q.T) / 2, 0)
+        Qs = Qs / Qs.sum(1, keepdims=True)
+        assert np.allclose(out[c], Qs @ (x[c] / cov[c][None, :]), rtol=1e-9)
+        assert np.allclose((out[c] @ np.linalg.pinv(x[c] / cov[c][None, :])).sum(1), 1.0)
+
+
+def test_avg_pool3_brute_force():
+    rng = np.random.default_rng(6)
+    x = rng.random((2, 5, 7))
+    got = R.avg_pool3(x)
+    for c in range(2):
+        for i in range(5):
+            for k in range(7):
+                s = sum(x[c, ii, kk] for ii in range(i - 1, i + 2) for kk in range(k - 1, k + 2)
+                        if 0 <= ii < 5 and 0 <= kk < 7)
+                assert abs(got[c, i, k] - s / 9.0) < 1e-12

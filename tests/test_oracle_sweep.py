@@ -1,0 +1,86 @@
+"""Pins the oracle's PARAFAC2 sweep (parafac2_intergrative.py:303-740) on a tiny dataset (CPU).
+
+* each contraction of one sweep against a float64 einsum written from the model
+  X_i ~ P_i B diag(A_i) C^T, C = M D;
+* the reference's error bookkeeping (norms, x_U, x_V, stale core norm) against the
+  directly evaluated PARAFAC2 loss;
+* the float32 restatement against the float64 one over several iterations."""
+import numpy as np
+import pytest
+
+from fasthigashi import synthetic as syn
+from oracle import ref_np as R
+from tests.helpers import build_dataset, canonical_corr, random_init
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    wl = syn.scaled(syn.WORKLOADS["c1"], n_cells=40, bins=[48, 36], rank=16, off_diag=10, band=10,
+                    name="sweep_tiny")
+    data, _, _ = build_dataset(wl, bs_bin=20, bs_cell=16, dt=np.float64)
+    F0, sizes = random_init(data, wl, wl.rank, seed=11)
+    return wl, data, F0.copy(np.float64), sizes
+
+
+def test_sweep_contractions(tiny):
+    wl, data, F, sizes = tiny
+    proj, projected, x_U, x_V, tnorm, M_new = R.sweep(data, F, True, np.float64)
+    C = [F.M @ F.D[c] for c in range(len(data))]
+    svd = np.zeros((wl.rank, F.M.shape[0]))
+    for c, cd in enumerate(data):
+        xu = 0.0
+        for t, tile in enumerate(cd.tiles):
+            X = cd.X[t]
+            A = F.A[c][tile.start:tile.start + tile.rows]
+            temp = np.einsum("iwk,ir,jr,kr->iwj", X, A, F.B[c], C[c])
+            # P_i is the polar factor of temp_i, and x_U sums its singular values
+            for i in range(tile.rows):
+                u, sv, vh = np.linalg.svd(temp[i], full_matrices=False)
+                assert np.allclose(proj[c][t][i], u @ vh, atol=1e-8)
+                xu += sv.sum()
+            P = proj[c][t]
+            svd += np.einsum("iwl,iwk,ir,lr,mr->mk", P, X, A, F.B[c], F.D[c])
+        assert np.isclose(x_U[c], xu, rtol=1e-10)
+        assert np.isclose(tnorm[c], sum((x ** 2).sum() for x in cd.X), rtol=1e-12)
+    u, sv, vh = np.linalg.svd(svd.T, full_matrices=False)
+    assert np.allclose(M_new, u @ vh, atol=1e-8)
+    assert np.isclose(x_V, sv.sum(), rtol=1e-10)
+    for c, cd in enumerate(data):
+        for t, tile in enumerate(cd.tiles):
+            want = np.einsum("iwl,iwk,km->ilm", proj[c][t], cd.X[t], M_new)
+            assert np.allclose(projected[c][tile.start:tile.start + tile.rows], want, rtol=1e-10, atol=1e-12)
+
+
+def test_error_bookkeeping_is_the_loss(tiny):
+    """err_V = ||X||^2 + core(A, B, D) - 2 x_V equals the PARAFAC2 loss at (P, A, B, D, M_new):
+    P and M_new have orthonormal columns, so ||P B diag(A_i) D^T M^T||^2 is the core norm."""
+    wl, data, F, sizes = tiny
+    proj, projected, x_U, x_V, tnorm, M_new = R.sweep(data, F, True, np.float64)
+    core = sum(R.core_norm(F.A[c], F.B[c], F.D[c], cd.bs_bin) for c, cd in enumerate(data))
+    G = F.copy(np.float64)
+    G.M = M_new
+    loss = R.parafac2_loss(data, G, proj)
+    assert np.isclose(tnorm.sum() + core - 2 * x_V, loss, rtol=1e-9)
+    # and the per-chromosome err_U uses x_U at the old M: loss with M replaced by the polar step's input
+    res = R.fit(data, F, 1, 1, 0.0, np.float64)
+    assert np.isclose(res.rec_errors[0] ** 2 * tnorm.sum(), loss, rtol=1e-9)
+
+
+def test_fit_decreases_and_fp32_tracks_fp64(tiny):
+    wl, data, F, sizes = tiny
+    r64 = R.fit(data, F, 6, 1, 0.0, np.float64)
+    r32 = R.fit(data, F.copy(np.float32), 6, 1, 0.0, np.float32)
+    e64 = np.array(r64.rec_errors)
+    assert all(b <= a * (1 + 1e-9) for a, b in zip(e64, e64[1:]))
+    assert np.max(np.abs(np.array(r32.rec_errors) - e64) / e64) < 1e-4
+    assert np.max(np.abs(np.array(r32.x_V) - np.array(r64.x_V)) / np.abs(np.array(r64.x_V))) < 1e-4
+    assert canonical_corr(r32.factors.M, r64.factors.M).min() > 0.999
+
+
+def test_transform_matches_last_sweep(tiny):
+    """transform recomputes SVD_term from the stored projections: with the factors the last
+    sweep used it reproduces that sweep's M."""
+    wl, data, F, sizes = tiny
+    proj, _, _, _, _, M_new = R.sweep(data, F, True, np.float64)
+    M_t = R.transform(data, F, proj, np.float64)
+    assert np.allclose(M_t, M_new, atol=1e-10)
