@@ -482,7 +482,240 @@ __global__ __launch_bounds__(64 * NsCfg<WT>::GM * NsCfg<WT>::GN) void ns_fused_k
 #undef ZB
 }
 
+// ---------------------------------------------------------------------------
+// Scaled Newton iteration for G^{-1/2} (s <= 160), one workgroup per matrix.
+//
+// With X = Y K the Newton step for the polar factor X <- (z X + z^-1 X^{-T}) / 2 only
+// needs H = X^T X:   W = (z I + z^-1 H^{-1}) / 2,  K <- K W,  H <- (z^2 H + 2 I + z^-2 H^{-1}) / 4,
+// with the Frobenius-norm scaling z = (tr H^{-1} / tr H)^{1/4} until ||H - I||_F < 0.1.
+// Starting from H = G / ||G||_F it converges in about log2(log(cond G)) + 3 steps; each
+// step is one Gauss-Jordan inverse (vector FP64, the matrix cyclically distributed over
+// the workgroup's registers, one barrier per pivot) and one s x s x s FP64 MFMA product.
+// A final Newton-Schulz correction K <- K (3 I - K H0 K) / 2 on the true residual removes
+// the drift of the H recurrence.  Output: G^{-1/2} in Z0, tr(G^{1/2}) in nuc.
+// ---------------------------------------------------------------------------
+template <int C> struct GjCfg;   // register grid of the in-place inverse: RG x CG threads
+template <> struct GjCfg<2> { static constexpr int RG = 16, CG = 16; };
+template <> struct GjCfg<4> { static constexpr int RG = 16, CG = 32; };
+template <> struct GjCfg<5> { static constexpr int RG = 20, CG = 32; };
+
+template <int NTH>
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  __syncthreads();                       // red may still be read by the previous reduction
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * (threadIdx.x >> 6)] = a;
+    red[2 * (threadIdx.x >> 6) + 1] = b;
+  }
+  __syncthreads();
+  a = 0.;
+  b = 0.;
+#pragma unroll
+  for (int w = 0; w < NTH / 64; ++w) {
+    a += red[2 * w];
+    b += red[2 * w + 1];
+  }
+}
+
+template <int WT>
+__global__ __launch_bounds__(64 * NsCfg<WT>::GM * NsCfg<WT>::GN) void polar_newton_kernel(
+    const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin, int max_iter,
+    double tol, double* __restrict__ nuc, int* __restrict__ iters) {
+  using Cfg = NsCfg<WT>;
+  constexpr int NTH = 64 * Cfg::GM * Cfg::GN;
+  constexpr int SMAX = 16 * Cfg::TM * Cfg::GM;
+  constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, RPT = SMAX / RG, CPT = SMAX / CG;
+  static_assert(RG * CG == NTH && RPT * RG == SMAX && CPT * CG == SMAX, "register grid must tile the matrix");
+  __shared__ double smem[2 * 16 * Cfg::TM * Cfg::GM * 17];
+  __shared__ double rowb[2][SMAX], colb[2][SMAX];
+  __shared__ double red[2 * (NTH / 64)];
+  const int64_t gm = blockIdx.x;
+  int lo = 0, hi = n_groups - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (mat_begin[mid] <= gm) lo = mid; else hi = mid - 1;
+  }
+  const FhNsGroup g = groups[lo];
+  const int64_t b = gm - mat_begin[lo];
+  const int64_t s = g.s, ld = g.ld;
+  if (ns_class(s) != WT) return;
+  double* const H0 = g.Y0 + b * g.stride;   // G in; scaled in place to G / c
+  double* const Hk = g.Z0 + b * g.stride;   // Newton iterate H, then the result
+  double* Wp = g.Y1 + b * g.stride;
+  double* Fp = g.Z1 + b * g.stride;
+  double* Xp = g.T + b * g.stride;
+  double* Kp = nullptr;                      // K = I until the first step
+  const int tid = threadIdx.x, ty = tid / CG, tx = tid % CG;
+
+  // c = ||G||_F,  H0 = Hk = G / c
+  double p0 = 0., p1 = 0.;
+  for (int64_t e = tid; e < s * s; e += NTH) {
+    const double v = H0[(e / s) * ld + e % s];
+    p0 += v * v;
+  }
+  block_sum2<NTH>(p0, p1, red);
+  double c = sqrt(p0);
+  if (!(c > 0.)) c = 1.;
+  const double inv_c = 1. / c;
+  for (int64_t e = tid; e < s * s; e += NTH) {
+    const int64_t r = e / s, q = e % s;
+    const double v = H0[r * ld + q] * inv_c;
+    H0[r * ld + q] = v;
+    Hk[r * ld + q] = v;
+  }
+  __threadfence_block();
+  __syncthreads();
+
+  double a[RPT][CPT];
+  bool scaling = true;
+  int it = 0;
+  while (it < max_iter) {
+    ++it;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) {
+        const int i = ty + RG * r, j = tx + CG * q;
+        a[r][q] = (i < s && j < s) ? Hk[i * ld + j] : (i == j ? 1. : 0.);
+      }
+    // Gauss-Jordan inverse in place (H is SPD: no pivoting); rows/columns >= s stay identity
+#pragma unroll 1
+    for (int k = 0; k < s; ++k) {
+      const int bf = k & 1;
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+        if (ty + RG * r == k) {
+#pragma unroll
+          for (int q = 0; q < CPT; ++q) rowb[bf][tx + CG * q] = a[r][q];
+        }
+#pragma unroll
+      for (int q = 0; q < CPT; ++q)
+        if (tx + CG * q == k) {
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) colb[bf][ty + RG * r] = a[r][q];
+        }
+      __syncthreads();
+      double piv = rowb[bf][k];
+      piv = piv > 1e-30 ? piv : 1e-30;       // numerically singular G: bounded stand-in pivot
+      const double ip = 1. / piv;
+      double cj[CPT];
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) cj[q] = rowb[bf][tx + CG * q] * ip;
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int i = ty + RG * r;
+        const double ri = colb[bf][i];
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+          const int j = tx + CG * q;
+          double v = a[r][q] - ri * cj[q];
+          v = (j == k) ? -ri * ip : v;
+          v = (i == k) ? ((j == k) ? ip : cj[q]) : v;
+          a[r][q] = v;
+        }
+      }
+    }
+    // scaling from the traces of H and H^{-1}
+    double th = 0., ti = 0.;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) {
+        const int i = ty + RG * r, j = tx + CG * q;
+        if (i == j && i < s) {
+          ti += a[r][q];
+          th += Hk[i * ld + i];
+        }
+      }
+    block_sum2<NTH>(th, ti, red);
+    const double z = scaling ? sqrt(sqrt(ti / th)) : 1.;
+    const double hz = 0.5 * z, hzi = 0.5 / z, qz2 = 0.25 * z * z, qzi2 = 0.25 / (z * z);
+    double dsq = 0., dummy = 0.;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) {
+        const int i = ty + RG * r, j = tx + CG * q;
+        if (i < s && j < s) {
+          const double hinv = a[r][q];
+          const double dg = (i == j) ? 1. : 0.;
+          Wp[i * ld + j] = hzi * hinv + hz * dg;
+          const double hn = qz2 * Hk[i * ld + j] + qzi2 * hinv + 0.5 * dg;
+          Hk[i * ld + j] = hn;
+          dsq += (hn - dg) * (hn - dg);
+        }
+      }
+    __threadfence_block();
+    block_sum2<NTH>(dsq, dummy, red);       // its barriers also publish W and H
+    if (Kp == nullptr) {                     // K_1 = W_0
+      Kp = Wp;
+      Wp = Fp;
+      Fp = Xp;
+    } else {                                 // K <- K W
+      full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Kp, Wp, Fp, s, ld, 1.0, 0.0, nullptr, smem);
+      __threadfence_block();
+      __syncthreads();
+      double* t = Kp;
+      Kp = Fp;
+      Fp = t;
+    }
+    if (dsq < 1e-2) scaling = false;
+    if (dsq < tol) break;
+  }
+  // correction on the true residual: K <- K (1.5 I - 0.5 K H0 K), result into Hk
+  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Kp, H0, Wp, s, ld, 1.0, 0.0, nullptr, smem);
+  __threadfence_block();
+  __syncthreads();
+  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Wp, Kp, Fp, s, ld, -0.5, 1.5, nullptr, smem);
+  __threadfence_block();
+  __syncthreads();
+  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Kp, Fp, Hk, s, ld, 1.0, 0.0, nullptr, smem);
+  __threadfence_block();
+  __syncthreads();
+  // G^{-1/2} = K / sqrt(c);  tr(G^{1/2}) = sum_ij G_ij (G^{-1/2})_ij
+  const double irs = 1. / sqrt(c);
+  double tr = 0., d2 = 0.;
+  for (int64_t e = tid; e < s * s; e += NTH) {
+    const int64_t r = e / s, q = e % s;
+    const double v = Hk[r * ld + q] * irs;
+    Hk[r * ld + q] = v;
+    tr += H0[r * ld + q] * v;
+  }
+  block_sum2<NTH>(tr, d2, red);
+  if (tid == 0) {
+    nuc[gm] = c * tr;
+    g.scale[b] = c;
+    if (iters) iters[gm] = it;
+  }
+}
+
 }  // namespace
+
+// Scaled Newton solve (s <= 160; larger s use the Newton-Schulz solve): G (in Y0) ->
+// G^{-1/2} in Z0, tr(G^{1/2}) in nuc, Newton steps in iters (nullable).  Y1, Z1, T are
+// scratch.  tol: stop when ||H - I||_F^2 < tol.
+extern "C" int fh_polar_newton(const FhNsGroup* groups_dev, int n_groups, const int64_t* mat_begin_dev,
+                               int64_t total_mats, int max_iter, double tol, double* nuc_dev, int* iters_dev,
+                               hipStream_t stream) {
+  FH_REQUIRE(max_iter >= 1 && max_iter <= 200, "polar_newton: max_iter out of range");
+  if (total_mats == 0) return FH_OK;
+  FH_REQUIRE(total_mats < (1ll << 31), "polar_newton: too many matrices");
+  const dim3 grid((unsigned)total_mats);
+  hipLaunchKernelGGL(polar_newton_kernel<5>, grid, dim3(640), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                     max_iter, tol, nuc_dev, iters_dev);
+  hipLaunchKernelGGL(polar_newton_kernel<4>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                     max_iter, tol, nuc_dev, iters_dev);
+  hipLaunchKernelGGL(polar_newton_kernel<2>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                     max_iter, tol, nuc_dev, iters_dev);
+  hipLaunchKernelGGL(ns_fused_kernel<0>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                     max_iter < 48 ? 48 : max_iter, 1e-24, nuc_dev, iters_dev);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
 
 extern "C" int64_t fh_ns_prepare(FhNsGroup* groups, int n) {
   int64_t total = 0;

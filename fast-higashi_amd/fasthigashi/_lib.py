@@ -79,6 +79,8 @@ _SIGS = {
     "fh_ns_run": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int64, c_int, c_double, c_void_p,
                           POINTER(c_int), c_void_p]),
     "fh_ns_fused": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int, c_double, c_void_p, c_void_p, c_void_p]),
+    "fh_polar_newton": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int, c_double, c_void_p, c_void_p,
+                                c_void_p]),
     "fh_spd_inverse": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_void_p]),
     "fh_fill": (c_int, [c_void_p, c_int, c_int64, c_uint64, c_void_p]),
     "fh_copy2d": (c_int, [c_void_p, c_int, c_int64, c_int64, c_void_p, c_int, c_int64, c_int64, c_int64,

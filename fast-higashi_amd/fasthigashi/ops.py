@@ -259,12 +259,22 @@ class PolarBatch:
         _, m = self.offs[g]
         return self.nuc[m: m + self.counts[g]]
 
-    def solve_fused(self, stream=None):
-        """One launch, one workgroup per matrix; G^{-1/2} ends in buffer 0."""
+    newton_iter = 24          # scaled Newton steps at most (about log2(log(cond G)) + 3 are used)
+    newton_tol = 1e-20        # stop when ||H - I||_F^2 < newton_tol
+
+    def solve_fused(self, stream=None, method="newton"):
+        """One workgroup per matrix; G^{-1/2} ends in buffer 0.  method 'newton': scaled Newton
+        with in-register Gauss-Jordan inverses (s <= 160, Newton-Schulz above); 'ns': coupled
+        Newton-Schulz for every size."""
         if not hasattr(self, "iters"):
             self.iters = torch.zeros(self.nmat, dtype=torch.int32, device=self.buf.device)
-        L.call("fh_ns_fused", self.table.data_ptr(), len(self.groups), self.mat_begin.data_ptr(), self.nmat,
-               self.n_iter, self.tol, self.nuc.data_ptr(), self.iters.data_ptr(), stream_handle(stream))
+        if method == "newton":
+            L.call("fh_polar_newton", self.table.data_ptr(), len(self.groups), self.mat_begin.data_ptr(),
+                   self.nmat, self.newton_iter, self.newton_tol, self.nuc.data_ptr(), self.iters.data_ptr(),
+                   stream_handle(stream))
+        else:
+            L.call("fh_ns_fused", self.table.data_ptr(), len(self.groups), self.mat_begin.data_ptr(), self.nmat,
+                   self.n_iter, self.tol, self.nuc.data_ptr(), self.iters.data_ptr(), stream_handle(stream))
         self.result = 0
 
     def solve(self, stream=None):
@@ -275,8 +285,9 @@ class PolarBatch:
         self.result = r.value
 
 
-def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24, return_gram: bool = False):
-    """Polar factor of a batch of fp32 matrices (b, m, s), m >= s, via FP64 Gram + Newton-Schulz.
+def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24, return_gram: bool = False, method="newton"):
+    """Polar factor of a batch of fp32 matrices (b, m, s), m >= s, via the FP64 Gram and its
+    inverse square root (scaled Newton, or Newton-Schulz with method='ns').
 
     Returns (P fp32 (b, m, s), nuclear norms fp64 (b,)[, FP64 Grams (b, s, ld)])."""
     squeeze = Y.dim() == 2
@@ -290,7 +301,7 @@ def polar(Y: torch.Tensor, n_iter: int = 48, tol: float = 1e-24, return_gram: bo
                   m * s, m * s, G.stride(0))
     Gemm64Group([d], DT_CODE[Yc.dtype], DT_CODE[Yc.dtype], L.F64).run()
     Gcopy = G.clone() if return_gram else None
-    pb.solve_fused()
+    pb.solve_fused(method=method)
     Z = pb.Zinv(0)
     P = torch.empty((b, m, s), dtype=torch.float32, device=Y.device)
     d2 = make_desc(Yc.data_ptr(), Z.data_ptr(), P.data_ptr(), m, s, s, s, 1, 1, Z.stride(1), s, 1, b, m * s,

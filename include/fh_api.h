@@ -127,6 +127,12 @@ int fh_ns_run(const FhNsGroup* groups_dev, int n_groups, const int64_t* mat_begi
  * nuc_dev[mat], iteration count into iters_dev[mat] */
 int fh_ns_fused(const FhNsGroup* groups_dev, int n_groups, const int64_t* mat_begin_dev, int64_t total_mats,
                 int max_iter, double tol, double* nuc_dev, int* iters_dev, fh_stream_t stream);
+/* scaled Newton iteration (s <= 160; larger s fall back to fh_ns_fused's tiled solve): per step one
+ * in-register Gauss-Jordan inverse of H = K^T G K and one K update, with Frobenius scaling
+ * z = (tr H^{-1} / tr H)^{1/4}; about log2(log(cond G)) + 3 steps instead of Newton-Schulz's
+ * O(log cond G).  Same buffers and outputs as fh_ns_fused; tol bounds ||H - I||_F^2. */
+int fh_polar_newton(const FhNsGroup* groups_dev, int n_groups, const int64_t* mat_begin_dev, int64_t total_mats,
+                    int max_iter, double tol, double* nuc_dev, int* iters_dev, fh_stream_t stream);
 
 /* ------------------------------------------------------------------ SPD inverse (FP64)
  * out = (G + eps I)^{-1} by Cholesky in LDS, one workgroup per system (s <= 256).

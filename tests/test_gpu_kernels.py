@@ -225,9 +225,41 @@ def test_polar_fused_matches_multilaunch(cuda):
         for pb in pbs:
             pb.G(g)[:, :, :s] = G
     pbs[0].solve()
-    pbs[1].solve_fused()
+    pbs[1].solve_fused(method="ns")
     for g, s in enumerate(sizes):
         a, b = pbs[0].Zinv(g).cpu().numpy(), pbs[1].Zinv(g).cpu().numpy()
         assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max()
     assert np.allclose(pbs[0].nuc.cpu().numpy(), pbs[1].nuc.cpu().numpy(), rtol=1e-12)
     assert (pbs[1].iters.cpu().numpy() < 48).all()
+
+
+@pytest.mark.parametrize("log_kappa", [0.5, 6.0, 11.0])
+def test_polar_newton_matches_eigh(cuda, log_kappa):
+    """Scaled Newton G^{-1/2} (every size class, plus the Newton-Schulz fallback above 160)
+    against the FP64 eigendecomposition, and its nuclear norm tr(G^{1/2})."""
+    from fasthigashi import ops
+    rng = np.random.default_rng(int(log_kappa * 10))
+    sizes, counts = [1, 9, 64, 65, 128, 129, 150, 160, 161], [2, 3, 2, 2, 1, 1, 2, 1, 1]
+    pb = ops.PolarBatch(sizes, counts, 48, 1e-24, cuda)
+    Gs = []
+    for g, (s, c) in enumerate(zip(sizes, counts)):
+        X = rng.standard_normal((c, s + 20, s)) * np.exp(rng.uniform(-log_kappa, 0, size=(c, 1, s)))
+        G = np.einsum("bmi,bmj->bij", X, X)
+        Gs.append(G)
+        pb.G(g)[:, :, :s] = torch.from_numpy(G).to(cuda)
+    pb.solve_fused(method="newton")
+    its = pb.iters.cpu().numpy()
+    for g, (s, G) in enumerate(zip(sizes, Gs)):
+        Z = pb.Zinv(g).cpu().numpy()[:, :, :s]
+        for b in range(G.shape[0]):
+            w, V = np.linalg.eigh(G[b])
+            ref = (V / np.sqrt(w)) @ V.T
+            cond = w.max() / w.min()
+            assert np.abs(Z[b] - ref).max() <= 1e-13 * cond * np.abs(ref).max(), (s, b, cond)
+            # the polar factor's orthogonality: Z G Z = I
+            assert np.abs(Z[b] @ G[b] @ Z[b] - np.eye(s)).max() <= 1e-14 * cond + 1e-12, (s, b)
+        nuc = pb.nuc_of(g).cpu().numpy()
+        ref_nuc = [np.sqrt(np.clip(np.linalg.eigvalsh(x), 0, None)).sum() for x in G]
+        assert np.allclose(nuc, ref_nuc, rtol=1e-11)
+    newton = np.concatenate([np.full(c, s <= 160) for s, c in zip(sizes, counts)])
+    assert (its[newton] <= 14).all(), its
