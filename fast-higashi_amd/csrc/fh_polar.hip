@@ -1,14 +1,15 @@
 // Batched FP64 polar factors and SPD solves.
 //
 // Polar factor of Y (m x s, m >= s):  polar(Y) = Y G^{-1/2},  G = Y^T Y (FP64).
-// G^{-1/2} comes from the coupled Newton-Schulz iteration on Y_0 = G/c, Z_0 = I
-// (c = ||G||_F >= lambda_max):
-//     T_k = 1.5 I - 0.5 Z_k Y_k,   Y_{k+1} = Y_k T_k,   Z_{k+1} = T_k Z_k
-// Y_k -> (G/c)^{1/2}, Z_k -> (G/c)^{-1/2}.  Every matrix of the batch iterates until
-// ||T_k - I||_F^2 < tol, then is frozen (copied through) so one fixed launch
-// sequence serves all of them.  Nuclear norm: trace(G^{1/2}) = sqrt(c) trace(Y).
-//
-// The iterations are s x s FP64 GEMMs on v_mfma_f64_16x16x4_f64, 64 x 64 tiles.
+// Two solvers for G^{-1/2}, both one workgroup per matrix:
+//  * scaled Newton (fh_polar_newton, s <= 160): H <- (z^2 H + 2 I + z^-2 H^{-1}) / 4 with an
+//    in-register Gauss-Jordan inverse per step, K <- K (z I + z^-1 H^{-1}) / 2 on the matrix
+//    cores; about 8 steps for any condition number;
+//  * coupled Newton-Schulz on Y_0 = G/c, Z_0 = I (c = ||G||_F >= lambda_max):
+//        T_k = 1.5 I - 0.5 Z_k Y_k,   Y_{k+1} = Y_k T_k,   Z_{k+1} = T_k Z_k
+//    Y_k -> (G/c)^{1/2}, Z_k -> (G/c)^{-1/2}, until ||T_k - I||_F^2 < tol; only products
+//    (v_mfma_f64_16x16x4_f64), O(log cond G) steps.  Used above s = 160 and by fh_ns_run.
+// Nuclear norm: trace(G^{1/2}).
 #include "fh_common.h"
 
 namespace {
@@ -290,9 +291,9 @@ __global__ __launch_bounds__(NT) void spd_inverse_kernel(const FhSpdSys* __restr
 // each owning TM x TN MFMA f64 16x16x4 tiles (s <= 16 TM GM = 16 TN GN), so every operand
 // element is loaded once per product.  Accumulates sum((out - I)^2) into *res if set.
 template <int GM, int GN, int TM, int TN>
-__device__ __attribute__((noinline)) void full_gemm_f64(const double* __restrict__ L, const double* __restrict__ R,
-                                              double* __restrict__ out, int64_t s, int64_t ld, double alpha,
-                                              double diag, double* res, double* smem) {
+__device__ __forceinline__ void full_gemm_f64_body(const double* __restrict__ L, const double* __restrict__ R,
+                                                   double* __restrict__ out, int64_t s, int64_t ld, double alpha,
+                                                   double diag, double* res, double* smem) {
   constexpr int TILE = 16 * TM * GM, BKD = 16, LR = BKD + 1, NTH = 64 * GM * GN;
   static_assert(16 * TN * GN == TILE, "square tile");
   double* As = smem;
@@ -363,6 +364,15 @@ __device__ __attribute__((noinline)) void full_gemm_f64(const double* __restrict
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     if (lane == 0) atomicAdd(res, part);
   }
+}
+
+// Out-of-line copy for callers with many call sites (keeps code size and register
+// pressure of the Newton-Schulz loop down).
+template <int GM, int GN, int TM, int TN>
+__device__ __attribute__((noinline)) void full_gemm_f64(const double* __restrict__ L, const double* __restrict__ R,
+                                              double* __restrict__ out, int64_t s, int64_t ld, double alpha,
+                                              double diag, double* res, double* smem) {
+  full_gemm_f64_body<GM, GN, TM, TN>(L, R, out, s, ld, alpha, diag, res, smem);
 }
 
 // wave grids of the fused solve per size class
@@ -483,21 +493,27 @@ __global__ __launch_bounds__(64 * NsCfg<WT>::GM * NsCfg<WT>::GN) void ns_fused_k
 }
 
 // ---------------------------------------------------------------------------
-// Scaled Newton iteration for G^{-1/2} (s <= 160), one workgroup per matrix.
+// Scaled Newton iteration for G^{-1/2} (s <= 160).
 //
 // With X = Y K the Newton step for the polar factor X <- (z X + z^-1 X^{-T}) / 2 only
 // needs H = X^T X:   W = (z I + z^-1 H^{-1}) / 2,  K <- K W,  H <- (z^2 H + 2 I + z^-2 H^{-1}) / 4,
 // with the Frobenius-norm scaling z = (tr H^{-1} / tr H)^{1/4} until ||H - I||_F < 0.1.
-// Starting from H = G / ||G||_F it converges in about log2(log(cond G)) + 3 steps; each
-// step is one Gauss-Jordan inverse (vector FP64, the matrix cyclically distributed over
-// the workgroup's registers, one barrier per pivot) and one s x s x s FP64 MFMA product.
-// A final Newton-Schulz correction K <- K (3 I - K H0 K) / 2 on the true residual removes
-// the drift of the H recurrence.  Output: G^{-1/2} in Z0, tr(G^{1/2}) in nuc.
+// Starting from H = G / ||G||_F it converges in about log2(log(cond G)) + 3 steps.  Each
+// step is two launches over the batch: the Gauss-Jordan inverse (vector FP64, the matrix
+// cyclically distributed over the workgroup's registers, one barrier per pivot) with the
+// H / W update, then K W on the matrix cores.  A final Newton-Schulz correction
+// K <- K (3 I - K H0 K) / 2 on the true (unshifted) residual removes the drift of the H
+// recurrence and the shift below.  Output: G^{-1/2} in Z0, tr(G^{1/2}) in nuc.
+//
+// Per-matrix state between launches lives in iters[]: bits 0-7 step count, bit 8 done,
+// bit 9 scaling off, bit 10 buffer (Z1 / T) holding K.
 // ---------------------------------------------------------------------------
-template <int C> struct GjCfg;   // register grid of the in-place inverse: RG x CG threads
-template <> struct GjCfg<2> { static constexpr int RG = 16, CG = 16; };
-template <> struct GjCfg<4> { static constexpr int RG = 16, CG = 32; };
-template <> struct GjCfg<5> { static constexpr int RG = 20, CG = 32; };
+// The iterate starts from G / ||G||_F + kNewtonShift I: rank-deficient Grams (which the
+// Newton-Schulz iteration tolerates) stay invertible; the final correction on the
+// unshifted G removes the shift's first-order effect on every direction with
+// eigenvalue >> kNewtonShift.
+constexpr double kNewtonShift = 1e-13;
+constexpr int kNtDone = 1 << 8, kNtNoScale = 1 << 9, kNtKbuf = 1 << 10, kNtCount = 0xff;
 
 template <int NTH>
 __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
@@ -521,198 +537,290 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
   }
 }
 
-template <int WT>
-__global__ __launch_bounds__(64 * NsCfg<WT>::GM * NsCfg<WT>::GN) void polar_newton_kernel(
-    const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin, int max_iter,
-    double tol, double* __restrict__ nuc, int* __restrict__ iters) {
-  using Cfg = NsCfg<WT>;
-  constexpr int NTH = 64 * Cfg::GM * Cfg::GN;
-  constexpr int SMAX = 16 * Cfg::TM * Cfg::GM;
-  constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, RPT = SMAX / RG, CPT = SMAX / CG;
-  static_assert(RG * CG == NTH && RPT * RG == SMAX && CPT * CG == SMAX, "register grid must tile the matrix");
-  __shared__ double smem[2 * 16 * Cfg::TM * Cfg::GM * 17];
-  __shared__ double rowb[2][SMAX], colb[2][SMAX];
-  __shared__ double red[2 * (NTH / 64)];
-  const int64_t gm = blockIdx.x;
+struct MatRef {
+  FhNsGroup g;
+  int64_t b;
+};
+__device__ __forceinline__ MatRef find_mat(const FhNsGroup* groups, int n_groups, const int64_t* mat_begin,
+                                           int64_t gm) {
   int lo = 0, hi = n_groups - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
     if (mat_begin[mid] <= gm) lo = mid; else hi = mid - 1;
   }
-  const FhNsGroup g = groups[lo];
-  const int64_t b = gm - mat_begin[lo];
-  const int64_t s = g.s, ld = g.ld;
-  if (ns_class(s) != WT) return;
-  double* const H0 = g.Y0 + b * g.stride;   // G in; scaled in place to G / c
-  double* const Hk = g.Z0 + b * g.stride;   // Newton iterate H, then the result
-  double* Wp = g.Y1 + b * g.stride;
-  double* Fp = g.Z1 + b * g.stride;
-  double* Xp = g.T + b * g.stride;
-  double* Kp = nullptr;                      // K = I until the first step
-  const int tid = threadIdx.x, ty = tid / CG, tx = tid % CG;
+  return MatRef{groups[lo], gm - mat_begin[lo]};
+}
 
-  // c = ||G||_F,  H0 = Hk = G / c
+// c = ||G||_F;  Y0 <- G / c;  Z0 <- G / c + shift I;  state cleared
+__global__ __launch_bounds__(256) void newton_init_kernel(const FhNsGroup* __restrict__ groups, int n_groups,
+                                                          const int64_t* __restrict__ mat_begin,
+                                                          int* __restrict__ state) {
+  __shared__ double red[8];
+  const MatRef m = find_mat(groups, n_groups, mat_begin, blockIdx.x);
+  const int64_t s = m.g.s, ld = m.g.ld;
+  if (ns_class(s) == 0) return;
+  double* H0 = m.g.Y0 + m.b * m.g.stride;
+  double* Hk = m.g.Z0 + m.b * m.g.stride;
   double p0 = 0., p1 = 0.;
-  for (int64_t e = tid; e < s * s; e += NTH) {
+  for (int64_t e = threadIdx.x; e < s * s; e += 256) {
     const double v = H0[(e / s) * ld + e % s];
     p0 += v * v;
   }
-  block_sum2<NTH>(p0, p1, red);
+  block_sum2<256>(p0, p1, red);
   double c = sqrt(p0);
   if (!(c > 0.)) c = 1.;
   const double inv_c = 1. / c;
-  for (int64_t e = tid; e < s * s; e += NTH) {
+  for (int64_t e = threadIdx.x; e < s * s; e += 256) {
     const int64_t r = e / s, q = e % s;
     const double v = H0[r * ld + q] * inv_c;
     H0[r * ld + q] = v;
-    Hk[r * ld + q] = v;
+    Hk[r * ld + q] = v + (r == q ? kNewtonShift : 0.);
   }
-  __threadfence_block();
-  __syncthreads();
+  if (threadIdx.x == 0) {
+    m.g.scale[m.b] = c;
+    state[blockIdx.x] = 0;
+  }
+}
 
+// register grid of the in-place inverse per size class: RG x CG threads, SMAX = matrix bound
+template <int C> struct GjCfg;
+template <> struct GjCfg<2> { static constexpr int RG = 16, CG = 16, SMAX = 64; };
+template <> struct GjCfg<4> { static constexpr int RG = 16, CG = 32, SMAX = 128; };
+template <> struct GjCfg<5> { static constexpr int RG = 16, CG = 32, SMAX = 160; };
+
+// One Newton step of every matrix of class WT that has not converged: H^{-1} by Gauss-Jordan,
+// the scaling, W (into Y1, or straight into Z1 as K at step 0) and the new H.
+//
+// Element (i, j) lives in a[i / RG][j / CG] of thread (i % RG, j % CG); the owner of a pivot
+// row / column follows from uniform arithmetic, so the only divergence is the owner's lane
+// mask.  Pivots are taken one at a time: the first step inverts G / ||G||_F + shift, whose
+// condition number reaches 1 / shift, and sequential elimination of an SPD matrix is
+// backward stable where an explicit inverse of a pivot block is not.  Rows/columns >= s are
+// identity padding.
+template <int WT>
+__global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void newton_gj_kernel(
+    const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin,
+    int* __restrict__ state, int step, double tol) {
+  constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX, NTH = RG * CG;
+  constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
+  static_assert(RPT * RG == SMAX && CPT * CG == SMAX, "register grid must tile the matrix");
+  __shared__ double rowb[2][SMAX], colb[2][SMAX];
+  __shared__ double red[2 * (NTH / 64)];
+  const MatRef m = find_mat(groups, n_groups, mat_begin, blockIdx.x);
+  const int s = (int)m.g.s, ld = (int)m.g.ld;
+  if (ns_class(s) != WT) return;
+  const int st = state[blockIdx.x];
+  if (st & kNtDone) return;
+  double* const Hk = m.g.Z0 + m.b * m.g.stride;
+  double* const Wp = (step == 0 ? m.g.Z1 : m.g.Y1) + m.b * m.g.stride;
+  const int tid = threadIdx.x, ty = tid / CG, tx = tid % CG;
   double a[RPT][CPT];
-  bool scaling = true;
-  int it = 0;
-  while (it < max_iter) {
-    ++it;
 #pragma unroll
-    for (int r = 0; r < RPT; ++r)
+  for (int r = 0; r < RPT; ++r)
 #pragma unroll
-      for (int q = 0; q < CPT; ++q) {
-        const int i = ty + RG * r, j = tx + CG * q;
-        a[r][q] = (i < s && j < s) ? Hk[i * ld + j] : (i == j ? 1. : 0.);
-      }
-    // Gauss-Jordan inverse in place (H is SPD: no pivoting); rows/columns >= s stay identity
+    for (int q = 0; q < CPT; ++q) {
+      const int i = ty + RG * r, j = tx + CG * q;
+      a[r][q] = (i < s && j < s) ? Hk[i * ld + j] : (i == j ? 1. : 0.);
+    }
 #pragma unroll 1
-    for (int k = 0; k < s; ++k) {
-      const int bf = k & 1;
+  for (int k = 0; k < s; ++k) {
+    const int bf = k & 1;
+    const int kr = k / RG, ky = k - kr * RG, kq = k / CG, kx = k - kq * CG;
+    if (ty == ky) {
 #pragma unroll
       for (int r = 0; r < RPT; ++r)
-        if (ty + RG * r == k) {
+        if (r == kr) {
 #pragma unroll
           for (int q = 0; q < CPT; ++q) rowb[bf][tx + CG * q] = a[r][q];
         }
+    }
+    if (tx == kx) {
 #pragma unroll
       for (int q = 0; q < CPT; ++q)
-        if (tx + CG * q == k) {
+        if (q == kq) {
 #pragma unroll
           for (int r = 0; r < RPT; ++r) colb[bf][ty + RG * r] = a[r][q];
         }
-      __syncthreads();
-      double piv = rowb[bf][k];
-      piv = piv > 1e-30 ? piv : 1e-30;       // numerically singular G: bounded stand-in pivot
-      const double ip = 1. / piv;
-      double cj[CPT];
-#pragma unroll
-      for (int q = 0; q < CPT; ++q) cj[q] = rowb[bf][tx + CG * q] * ip;
-#pragma unroll
-      for (int r = 0; r < RPT; ++r) {
-        const int i = ty + RG * r;
-        const double ri = colb[bf][i];
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) {
-          const int j = tx + CG * q;
-          double v = a[r][q] - ri * cj[q];
-          v = (j == k) ? -ri * ip : v;
-          v = (i == k) ? ((j == k) ? ip : cj[q]) : v;
-          a[r][q] = v;
-        }
-      }
     }
-    // scaling from the traces of H and H^{-1}
-    double th = 0., ti = 0.;
+    __syncthreads();
+    double piv = rowb[bf][k];
+    piv = piv > 1e-300 ? piv : 1e-300;
+    const double ip = 1. / piv;
+    double v[CPT], u[RPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) v[q] = rowb[bf][tx + CG * q] * ip;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) u[r] = colb[bf][ty + RG * r];
+    // Schur update everywhere, then row k <- A[k, :] / p, column k <- -A[:, k] / p, (k, k) <- 1 / p
 #pragma unroll
     for (int r = 0; r < RPT; ++r)
 #pragma unroll
-      for (int q = 0; q < CPT; ++q) {
-        const int i = ty + RG * r, j = tx + CG * q;
-        if (i == j && i < s) {
-          ti += a[r][q];
-          th += Hk[i * ld + i];
-        }
-      }
-    block_sum2<NTH>(th, ti, red);
-    const double z = scaling ? sqrt(sqrt(ti / th)) : 1.;
-    const double hz = 0.5 * z, hzi = 0.5 / z, qz2 = 0.25 * z * z, qzi2 = 0.25 / (z * z);
-    double dsq = 0., dummy = 0.;
+      for (int q = 0; q < CPT; ++q) a[r][q] = fma(-u[r], v[q], a[r][q]);
+    if (ty == ky) {
 #pragma unroll
-    for (int r = 0; r < RPT; ++r)
+      for (int r = 0; r < RPT; ++r)
+        if (r == kr) {
 #pragma unroll
-      for (int q = 0; q < CPT; ++q) {
-        const int i = ty + RG * r, j = tx + CG * q;
-        if (i < s && j < s) {
-          const double hinv = a[r][q];
-          const double dg = (i == j) ? 1. : 0.;
-          Wp[i * ld + j] = hzi * hinv + hz * dg;
-          const double hn = qz2 * Hk[i * ld + j] + qzi2 * hinv + 0.5 * dg;
-          Hk[i * ld + j] = hn;
-          dsq += (hn - dg) * (hn - dg);
+          for (int q = 0; q < CPT; ++q) a[r][q] = v[q];
         }
-      }
-    __threadfence_block();
-    block_sum2<NTH>(dsq, dummy, red);       // its barriers also publish W and H
-    if (Kp == nullptr) {                     // K_1 = W_0
-      Kp = Wp;
-      Wp = Fp;
-      Fp = Xp;
-    } else {                                 // K <- K W
-      full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Kp, Wp, Fp, s, ld, 1.0, 0.0, nullptr, smem);
-      __threadfence_block();
-      __syncthreads();
-      double* t = Kp;
-      Kp = Fp;
-      Fp = t;
     }
-    if (dsq < 1e-2) scaling = false;
-    if (dsq < tol) break;
+    if (tx == kx) {
+#pragma unroll
+      for (int q = 0; q < CPT; ++q)
+        if (q == kq) {
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) a[r][q] = (ty == ky && r == kr) ? ip : -u[r] * ip;
+        }
+    }
   }
-  // correction on the true residual: K <- K (1.5 I - 0.5 K H0 K), result into Hk
-  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Kp, H0, Wp, s, ld, 1.0, 0.0, nullptr, smem);
+  // scaling from the traces of H and H^{-1}
+  double th = 0., ti = 0.;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int i = ty + RG * r, j = tx + CG * q;
+      if (i == j && i < s) {
+        ti += a[r][q];
+        th += Hk[i * ld + i];
+      }
+    }
+  block_sum2<NTH>(th, ti, red);
+  const double ratio = ti / th;
+  const bool scaling = !(st & kNtNoScale);
+  const double z = (scaling && ratio > 0. && ratio < 1e300) ? sqrt(sqrt(ratio)) : 1.;
+  const double hz = 0.5 * z, hzi = 0.5 / z, qz2 = 0.25 * z * z, qzi2 = 0.25 / (z * z);
+  double dsq = 0., dummy = 0.;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int i = ty + RG * r, j = tx + CG * q;
+      if (i < s && j < s) {
+        const double hinv = a[r][q];
+        const double dg = (i == j) ? 1. : 0.;
+        const int o = i * ld + j;
+        Wp[o] = hzi * hinv + hz * dg;
+        const double hn = qz2 * Hk[o] + qzi2 * hinv + 0.5 * dg;
+        Hk[o] = hn;
+        dsq += (hn - dg) * (hn - dg);
+      }
+    }
+  block_sum2<NTH>(dsq, dummy, red);
+  if (tid == 0) {
+    // K after this step: Z1 at step 0, then alternating T / Z1
+    int ns = (st & ~kNtCount) | (step + 1);
+    if (dsq < 1e-2) ns |= kNtNoScale;
+    if (dsq < tol) ns |= kNtDone;
+    state[blockIdx.x] = ns;
+  }
+}
+
+// K <- K W for the matrices stepped in launch `step` (>= 1): K in Z1 (odd step) / T (even step)
+template <int WT>
+__global__ __launch_bounds__(64 * NsCfg<WT>::GM * NsCfg<WT>::GN) void newton_kw_kernel(
+    const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin,
+    int* __restrict__ state, int step) {
+  using Cfg = NsCfg<WT>;
+  __shared__ double smem[2 * 16 * Cfg::TM * Cfg::GM * 17];
+  const MatRef m = find_mat(groups, n_groups, mat_begin, blockIdx.x);
+  const int64_t s = m.g.s, ld = m.g.ld;
+  if (ns_class(s) != WT) return;
+  const int st = state[blockIdx.x];
+  if ((st & kNtCount) != step + 1) return;     // converged before this step
+  double* const K0 = m.g.Z1 + m.b * m.g.stride;
+  double* const K1 = m.g.T + m.b * m.g.stride;
+  const double* W = m.g.Y1 + m.b * m.g.stride;
+  const bool from_z1 = (step & 1);             // K of step-1 in Z1 when step is odd
+  full_gemm_f64_body<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(from_z1 ? K0 : K1, W, from_z1 ? K1 : K0, s, ld, 1.0,
+                                                         0.0, nullptr, smem);
+  if (threadIdx.x == 0) state[blockIdx.x] = from_z1 ? (st | kNtKbuf) : (st & ~kNtKbuf);
+}
+
+// Correction on the true residual, K <- K (1.5 I - 0.5 K H0 K), then G^{-1/2} = K / sqrt(c)
+// into Z0 and tr(G^{1/2}) = sum_ij G_ij (G^{-1/2})_ij.
+template <int WT>
+__global__ __launch_bounds__(64 * NsCfg<WT>::GM * NsCfg<WT>::GN) void newton_final_kernel(
+    const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin,
+    int* __restrict__ state, double* __restrict__ nuc) {
+  using Cfg = NsCfg<WT>;
+  constexpr int NTH = 64 * Cfg::GM * Cfg::GN;
+  __shared__ double smem[2 * 16 * Cfg::TM * Cfg::GM * 17];
+  __shared__ double red[2 * (NTH / 64)];
+  const MatRef m = find_mat(groups, n_groups, mat_begin, blockIdx.x);
+  const int64_t s = m.g.s, ld = m.g.ld;
+  if (ns_class(s) != WT) return;
+  const int st = state[blockIdx.x];
+  double* const H0 = m.g.Y0 + m.b * m.g.stride;
+  double* const Out = m.g.Z0 + m.b * m.g.stride;
+  double* const Sc = m.g.Y1 + m.b * m.g.stride;
+  double* const K = ((st & kNtKbuf) ? m.g.T : m.g.Z1) + m.b * m.g.stride;
+  double* const M = ((st & kNtKbuf) ? m.g.Z1 : m.g.T) + m.b * m.g.stride;
+  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(K, H0, Sc, s, ld, 1.0, 0.0, nullptr, smem);
   __threadfence_block();
   __syncthreads();
-  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Wp, Kp, Fp, s, ld, -0.5, 1.5, nullptr, smem);
+  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Sc, K, M, s, ld, -0.5, 1.5, nullptr, smem);
   __threadfence_block();
   __syncthreads();
-  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(Kp, Fp, Hk, s, ld, 1.0, 0.0, nullptr, smem);
+  full_gemm_f64<Cfg::GM, Cfg::GN, Cfg::TM, Cfg::TN>(K, M, Out, s, ld, 1.0, 0.0, nullptr, smem);
   __threadfence_block();
   __syncthreads();
-  // G^{-1/2} = K / sqrt(c);  tr(G^{1/2}) = sum_ij G_ij (G^{-1/2})_ij
+  const double c = m.g.scale[m.b];
   const double irs = 1. / sqrt(c);
   double tr = 0., d2 = 0.;
-  for (int64_t e = tid; e < s * s; e += NTH) {
+  for (int64_t e = threadIdx.x; e < s * s; e += NTH) {
     const int64_t r = e / s, q = e % s;
-    const double v = Hk[r * ld + q] * irs;
-    Hk[r * ld + q] = v;
+    const double v = Out[r * ld + q] * irs;
+    Out[r * ld + q] = v;
     tr += H0[r * ld + q] * v;
   }
   block_sum2<NTH>(tr, d2, red);
-  if (tid == 0) {
-    nuc[gm] = c * tr;
-    g.scale[b] = c;
-    if (iters) iters[gm] = it;
+  if (threadIdx.x == 0) {
+    nuc[blockIdx.x] = c * tr;
+    state[blockIdx.x] = st & kNtCount;
   }
 }
 
 }  // namespace
 
 // Scaled Newton solve (s <= 160; larger s use the Newton-Schulz solve): G (in Y0) ->
-// G^{-1/2} in Z0, tr(G^{1/2}) in nuc, Newton steps in iters (nullable).  Y1, Z1, T are
-// scratch.  tol: stop when ||H - I||_F^2 < tol.
+// G^{-1/2} in Z0, tr(G^{1/2}) in nuc, Newton steps in iters (required: it also carries the
+// per-matrix state between the launches).  Y1, Z1, T are scratch.  max_iter launches of
+// each step kernel; tol: a matrix stops when ||H - I||_F^2 < tol.
 extern "C" int fh_polar_newton(const FhNsGroup* groups_dev, int n_groups, const int64_t* mat_begin_dev,
                                int64_t total_mats, int max_iter, double tol, double* nuc_dev, int* iters_dev,
                                hipStream_t stream) {
-  FH_REQUIRE(max_iter >= 1 && max_iter <= 200, "polar_newton: max_iter out of range");
+  FH_REQUIRE(max_iter >= 1 && max_iter <= 64, "polar_newton: max_iter out of range (1..64)");
+  FH_REQUIRE(iters_dev != nullptr, "polar_newton: iters (the per-matrix state) is required");
   if (total_mats == 0) return FH_OK;
   FH_REQUIRE(total_mats < (1ll << 31), "polar_newton: too many matrices");
   const dim3 grid((unsigned)total_mats);
-  hipLaunchKernelGGL(polar_newton_kernel<5>, grid, dim3(640), 0, stream, groups_dev, n_groups, mat_begin_dev,
-                     max_iter, tol, nuc_dev, iters_dev);
-  hipLaunchKernelGGL(polar_newton_kernel<4>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev,
-                     max_iter, tol, nuc_dev, iters_dev);
-  hipLaunchKernelGGL(polar_newton_kernel<2>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
-                     max_iter, tol, nuc_dev, iters_dev);
-  hipLaunchKernelGGL(ns_fused_kernel<0>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
-                     max_iter < 48 ? 48 : max_iter, 1e-24, nuc_dev, iters_dev);
+  hipLaunchKernelGGL(newton_init_kernel, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                     iters_dev);
+  for (int step = 0; step < max_iter; ++step) {
+    hipLaunchKernelGGL(newton_gj_kernel<5>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                       iters_dev, step, tol);
+    hipLaunchKernelGGL(newton_gj_kernel<4>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                       iters_dev, step, tol);
+    hipLaunchKernelGGL(newton_gj_kernel<2>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                       iters_dev, step, tol);
+    if (step > 0) {
+      hipLaunchKernelGGL(newton_kw_kernel<5>, grid, dim3(640), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                         iters_dev, step);
+      hipLaunchKernelGGL(newton_kw_kernel<4>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                         iters_dev, step);
+      hipLaunchKernelGGL(newton_kw_kernel<2>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                         iters_dev, step);
+    }
+  }
+  hipLaunchKernelGGL(newton_final_kernel<5>, grid, dim3(640), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                     iters_dev, nuc_dev);
+  hipLaunchKernelGGL(newton_final_kernel<4>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                     iters_dev, nuc_dev);
+  hipLaunchKernelGGL(newton_final_kernel<2>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
+                     iters_dev, nuc_dev);
+  hipLaunchKernelGGL(ns_fused_kernel<0>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev, 48,
+                     1e-24, nuc_dev, iters_dev);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }

@@ -259,7 +259,7 @@ class PolarBatch:
         _, m = self.offs[g]
         return self.nuc[m: m + self.counts[g]]
 
-    newton_iter = 24          # scaled Newton steps at most (about log2(log(cond G)) + 3 are used)
+    newton_iter = 12          # scaled Newton steps at most (about log2(log(cond G)) + 3 are used)
     newton_tol = 1e-20        # stop when ||H - I||_F^2 < newton_tol
 
     def solve_fused(self, stream=None, method="newton"):
