@@ -163,6 +163,46 @@ __global__ void diag_kernel(float* X, int64_t nb, int64_t stride, int64_t ld, in
 //   loss = ||T||^2 - 2 <M_D, D> + ||[[A,B,D]]||^2 ; wdiff = (prev - loss) / prev; stop if wdiff < 1e-5
 // state: [0] = previous loss, [1] = have previous; flag: 1 once stopped.
 
+
+// Grouped batched transpose: dst[b][c][r] = src[b][r][c] (r < rows, c < cols) with fp32/bf16
+// conversion; one 64 x 64 tile per workgroup, staged through LDS so both sides are coalesced.
+__global__ __launch_bounds__(NT) void transpose_grouped_kernel(const FhTransDesc* __restrict__ descs, int n) {
+  __shared__ float tile[64][65];
+  const int64_t t = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
+  }
+  const FhTransDesc d = descs[lo];
+  int64_t local = t - d.tile_begin;
+  const int64_t per = (int64_t)d.tiles_r * d.tiles_c;
+  const int64_t b = local / per;
+  local -= b * per;
+  const int64_t r0 = (local / d.tiles_c) * 64, c0 = (local % d.tiles_c) * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const char* src = (const char*)d.src + b * d.sbs * (d.sdt == FH_BF16 ? 2 : 4);
+  char* dst = (char*)d.dst + b * d.sbd * (d.ddt == FH_BF16 ? 2 : 4);
+  for (int i = ty; i < 64; i += NT / 64) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    float v = 0.f;
+    if (r < d.rows && c < d.cols) {
+      const int64_t o = r * d.lds + c;
+      v = d.sdt == FH_BF16 ? (float)((const __bf16*)src)[o] : ((const float*)src)[o];
+    }
+    tile[i][tx] = v;
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += NT / 64) {
+    const int64_t c = c0 + i, r = r0 + tx;
+    if (r < d.rows && c < d.cols) {
+      const int64_t o = c * d.ldd + r;
+      const float v = tile[tx][i];
+      if (d.ddt == FH_BF16) ((uint16_t*)dst)[o] = fh_f32_to_bf16_bits(v);
+      else ((float*)dst)[o] = v;
+    }
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -249,6 +289,15 @@ int fh_batch_sum(const void* in, int dt, int64_t nb, int64_t stride, int64_t cou
 int fh_diag(float* X, int64_t nb, int64_t stride, int64_t ld, int64_t n, float v, int add, hipStream_t st) {
   if (nb * n <= 0) return FH_OK;
   hipLaunchKernelGGL(diag_kernel, dim3(grid_for(nb * n)), dim3(NT), 0, st, X, nb, stride, ld, n, v, add);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+
+int fh_transpose_grouped(const FhTransDesc* descs_dev, int n_desc, int64_t total_tiles, hipStream_t st) {
+  if (total_tiles <= 0) return FH_OK;
+  FH_REQUIRE(total_tiles < (1ll << 31), "transpose_grouped: too many tiles");
+  hipLaunchKernelGGL(transpose_grouped_kernel, dim3((unsigned)total_tiles), dim3(NT), 0, st, descs_dev, n_desc);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }

@@ -56,6 +56,13 @@ class BandTile(Structure):
                 ("row_start", c_int32), ("rows", c_int32), ("shift", c_int32), ("width", c_int32)]
 
 
+class TransDesc(Structure):
+    _fields_ = [("src", c_void_p), ("dst", c_void_p),
+                ("rows", c_int64), ("cols", c_int64), ("lds", c_int64), ("ldd", c_int64), ("sbs", c_int64),
+                ("sbd", c_int64), ("batch", c_int32), ("sdt", c_int32), ("ddt", c_int32), ("pad", c_int32),
+                ("tile_begin", c_int64), ("tiles_r", c_int32), ("tiles_c", c_int32)]
+
+
 class CpChrom(Structure):
     _fields_ = [("A", c_void_p), ("B", c_void_p), ("D", c_void_p), ("T", c_void_p), ("W1", c_void_p),
                 ("MF", c_void_p), ("KR", c_void_p), ("MDpart", c_void_p), ("G", c_void_p), ("scal", c_void_p),
@@ -92,6 +99,7 @@ _SIGS = {
     "fh_segment_sum": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "fh_sweep_errors": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "fh_batch_sum": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "fh_transpose_grouped": (c_int, [c_void_p, c_int, c_int64, c_void_p]),
     "fh_diag": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_float, c_int, c_void_p]),
     "fh_cpals_op": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_void_p]),
     "fh_cpals_tnorm": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p]),

@@ -276,7 +276,22 @@ class SweepEngine:
         self.Y = z(yoff)
         self.temp = z(yoff)
         self.P = z(yoff)
-        self.P16 = z(yoff, dt=torch.bfloat16) if dtype == "bf16" else None
+        # bf16 path: k-contiguous copies for T_hat = P^T X (per bin: X_i^T cells x positions,
+        # P_i^T sp x positions, positions padded to a multiple of 8)
+        self.P16 = None
+        self.Xt = self.P16t = None
+        if dtype == "bf16":
+            xt = pt = 0
+            for cl in self.chroms:
+                cl.xtoff, cl.ptoff = [], []
+                for t in cl.tiles:
+                    wp = rup(t.width, 8)
+                    cl.xtoff.append(xt)
+                    cl.ptoff.append(pt)
+                    xt += t.rows * self.Npad * wp
+                    pt += t.rows * cl.sp * wp
+            self.Xt = z(xt, dt=torch.bfloat16)
+            self.P16t = z(pt, dt=torch.bfloat16)
         self.That = z(self.Ttot, self.Npad, dt=self.tdtype)
         self.Lk = z(loff, dt=self.tdtype)
         self.Zt = z(self.Npad, self.Dld)
@@ -360,7 +375,10 @@ class SweepEngine:
         self.finish_X()
 
     def finish_X(self):
-        """Squared norms of each chromosome's X (the reference's first-sweep ||X||^2)."""
+        """Squared norms of each chromosome's X (the reference's first-sweep ||X||^2); on the
+        bf16 path also the per-bin transposed copy X_i^T used by T_hat = P^T X."""
+        if self.Xt is not None:
+            self.g_xt.run()
         rpb = 64
         for c, cl in enumerate(self.chroms):
             X = self.X[cl.xrow: cl.xrow + cl.W]
@@ -431,16 +449,36 @@ class SweepEngine:
                                     tl.width, cl.s, cl.s, cl.sp, 1, Z.stride(1), 1, cl.sp, 1, tl.rows,
                                     tl.width * cl.sp, Z.stride(0), tl.width * cl.sp))
         self.g_p = ops.Gemm64Group(d7, F32, F64, F32, dev)
-        # (8) T_hat_i = P_i^T X_i
-        Pp = self.P16.data_ptr() if self.dtype == "bf16" else self.P.data_ptr()
+        # (8) T_hat_i = P_i^T X_i.  bf16: both operands k-contiguous (P_i^T and X_i^T copies);
+        # fp32: P_i and X_i as stored (m- and n-contiguous)
         d8 = []
-        for c, cl in enumerate(self.chroms):
-            for t, tl in enumerate(cl.tiles):
-                o = cl.yoff + cl.trow[t] * cl.sp
-                d8.append(make_desc(Pp + xe * o, Xp + xe * (cl.xrow + cl.trow[t]) * Np,
-                                    Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
-                                    cl.s, Np, tl.width, 1, cl.sp, Np, 1, Np, 1, tl.rows, tl.width * cl.sp,
-                                    tl.width * Np, cl.sp * Np, flags=out16))
+        if self.dtype == "bf16":
+            xt = []
+            pt = []
+            for c, cl in enumerate(self.chroms):
+                for t, tl in enumerate(cl.tiles):
+                    wp = rup(tl.width, 8)
+                    xo = (cl.xrow + cl.trow[t]) * Np
+                    xt.append((Xp + 2 * xo, BF16, self.Xt.data_ptr() + 2 * cl.xtoff[t], BF16, tl.width, Np, Np,
+                               wp, tl.rows, tl.width * Np, Np * wp))
+                    po = cl.yoff + cl.trow[t] * cl.sp
+                    pt.append((f4(self.P, po), F32, self.P16t.data_ptr() + 2 * cl.ptoff[t], BF16, tl.width,
+                               cl.s, cl.sp, wp, tl.rows, tl.width * cl.sp, cl.sp * wp))
+                    d8.append(make_desc(self.P16t.data_ptr() + 2 * cl.ptoff[t],
+                                        self.Xt.data_ptr() + 2 * cl.xtoff[t],
+                                        Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
+                                        cl.s, Np, tl.width, wp, 1, 1, wp, Np, 1, tl.rows, cl.sp * wp,
+                                        Np * wp, cl.sp * Np, flags=out16))
+            self.g_xt = ops.TransposeGroup(xt, dev)
+            self.g_pt = ops.TransposeGroup(pt, dev)
+        else:
+            for c, cl in enumerate(self.chroms):
+                for t, tl in enumerate(cl.tiles):
+                    o = cl.yoff + cl.trow[t] * cl.sp
+                    d8.append(make_desc(f4(self.P, o), Xp + xe * (cl.xrow + cl.trow[t]) * Np,
+                                        Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
+                                        cl.s, Np, tl.width, 1, cl.sp, Np, 1, Np, 1, tl.rows, tl.width * cl.sp,
+                                        tl.width * Np, cl.sp * Np, flags=out16))
         self.g_that = MultiGemm(d8, self.xdt, lambda d: shape_for_m(d.M), dev)
         # (10) Zt_c = T_hat_c^T L_c^T
         d10 = []
@@ -545,8 +583,7 @@ class SweepEngine:
         self.g_p.run(stream)
         L.call("fh_segment_sum", self.pb.nuc.data_ptr(), self.seg.data_ptr(), self.K, self.xU.data_ptr(), 0, st)
         if self.dtype == "bf16":
-            L.call("fh_copy2d", self.P.data_ptr(), F32, self.P.numel(), 1, self.P16.data_ptr(), BF16,
-                   self.P.numel(), 1, self.P.numel(), 1.0, st)
+            self.g_pt.run(stream)
         # pass 2
         mk("that")
         self.g_that.run(stream)
@@ -578,8 +615,7 @@ class SweepEngine:
         st = ops.stream_handle(stream)
         M_keep = self.M.clone()
         if self.dtype == "bf16":
-            L.call("fh_copy2d", self.P.data_ptr(), F32, self.P.numel(), 1, self.P16.data_ptr(), BF16,
-                   self.P.numel(), 1, self.P.numel(), 1.0, st)
+            self.g_pt.run(stream)
         self.g_that.run(stream)
         for c, cl in enumerate(self.chroms):
             L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,

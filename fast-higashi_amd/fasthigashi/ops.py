@@ -65,6 +65,32 @@ def _upload(structs, device) -> torch.Tensor:
     return host.to(device)
 
 
+class TransposeGroup:
+    """One launch of the grouped batched transpose dst[b][c][r] = src[b][r][c].
+
+    items: (src_ptr, src_dtype, dst_ptr, dst_dtype, rows, cols, lds, ldd, batch, src_batch_stride,
+    dst_batch_stride), strides in elements."""
+
+    def __init__(self, items, device="cuda"):
+        descs = []
+        total = 0
+        for (sp_, sdt, dp, ddt, rows, cols, lds, ldd, batch, sbs, sbd) in items:
+            d = L.TransDesc()
+            d.src, d.dst, d.sdt, d.ddt = sp_, dp, sdt, ddt
+            d.rows, d.cols, d.lds, d.ldd, d.batch, d.sbs, d.sbd = rows, cols, lds, ldd, batch, sbs, sbd
+            assert cols <= lds and rows <= ldd, "transpose: leading dims shorter than the extents"
+            d.tiles_r, d.tiles_c = -(-rows // 64), -(-cols // 64)
+            d.tile_begin = total
+            total += batch * d.tiles_r * d.tiles_c
+            descs.append(d)
+        self.total = total
+        self.n = len(descs)
+        self.table = _upload((L.TransDesc * len(descs))(*descs), device)
+
+    def run(self, stream=None):
+        L.call("fh_transpose_grouped", self.table.data_ptr(), self.n, self.total, stream_handle(stream))
+
+
 class GemmGroup:
     """One grouped launch of the fp32/bf16/fp8 MFMA GEMM over a fixed descriptor table."""
 

@@ -248,6 +248,20 @@ int fh_sweep_errors(const double* tnorm, const double* core, const double* xU, c
 /* out[i] = sum_b in[b*stride + i] in fixed order (split-K reductions, e.g. SVD_term :430) */
 int fh_batch_sum(const void* in, int dt, int64_t nb, int64_t stride, int64_t count, void* out,
                  fh_stream_t stream);
+/* grouped batched transpose dst[b][c][r] = src[b][r][c] with fp32 / bf16 conversion, one launch for
+ * a table of descriptors (tile_begin: prefix sum of batch * tiles_r * tiles_c, 64 x 64 tiles).
+ * Keeps both operands of the P^T X product k-contiguous (the positions x cells tiles of
+ * fetch(transpose=True), sparse_for_schic.py:588-613, and the projections of
+ * parafac2_intergrative.py:429). */
+typedef struct FhTransDesc {
+  const void* src;
+  void* dst;
+  int64_t rows, cols, lds, ldd, sbs, sbd;
+  int32_t batch, sdt, ddt, pad;
+  int64_t tile_begin;
+  int32_t tiles_r, tiles_c;
+} FhTransDesc;
+int fh_transpose_grouped(const FhTransDesc* descs_dev, int n_desc, int64_t total_tiles, fh_stream_t stream);
 /* X[b][i][i] = v (add = 0) or += v (add = 1): torch.eye / diagonal add, partial_rwr.py:113, 118 */
 int fh_diag(float* X, int64_t nb, int64_t stride, int64_t ld, int64_t n, float v, int add, fh_stream_t stream);
 
