@@ -28,42 +28,49 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def phase_work(eng, phase, polar_iters):
-    """Algorithmic work of one launch of a sweep phase: (bound, amount, unit-per-second name).
+def launch_work(eng, phase):
+    """Algorithmic work of one single-launch phase: (bound, bytes, flops, description).
 
-    mttkrp_y:  X_c C_c              2 W_c N s_c flop; streams X once (W_c N bytes)
-    that:      P_i^T X_i            2 W_c N s_c flop; streams X once
-    z:         T_hat^T L^T          2 n_c s_c N s_c flop; streams T_hat once
-    ttilde:    T_hat M              2 n_c s_c N R flop; streams T_hat once
-    polar_ns:  Newton-Schulz        sum over matrices of iterations * 3 * 2 s^3 flop (FP64)
-    """
-    N, R = eng.N, eng.R
+    'that:<shape>' is one grouped launch of T_hat_i = P_i^T X_i over every bin whose
+    chromosome uses that tile shape: per bin 2 s w N flops; bytes = X_i (w x N) +
+    P_i (s x w) read once, T_hat_i (s x N) written once, at the path's element size.
+    Padding (cells up to a multiple of 64, positions up to a multiple of 8) is not counted."""
+    if not phase.startswith("that:"):
+        return None
+    shape = int(phase.split(":")[1])
     xe = 2 if eng.dtype == "bf16" else 4
-    if phase in ("mttkrp_y", "that"):
-        fl = sum(2.0 * c.W * N * c.s for c in eng.chroms)
-        by = sum(float(c.W) * eng.Npad * xe for c in eng.chroms)
-    elif phase == "z":
-        fl = sum(2.0 * c.n * c.s * N * c.s for c in eng.chroms)
-        by = sum(float(c.n * c.sp) * eng.Npad * xe for c in eng.chroms)
-    elif phase == "ttilde":
-        fl = sum(2.0 * c.n * c.s * N * R for c in eng.chroms)
-        by = sum(float(c.n * c.sp) * eng.Npad * xe for c in eng.chroms)
-    elif phase == "polar_ns":
-        fl = 0.0
-        off = 0
-        for c in eng.chroms:
-            it = polar_iters[off: off + c.n].astype("float64")
-            fl += float((it + 1).sum()) * 2.0 * c.s ** 3 + float(it.sum()) * 4.0 * c.s ** 3
-            off += c.n
-        return "mfma", fl, "fp64"
-    else:
-        return None, 0.0, None
-    peak_key = "bf16" if eng.dtype == "bf16" else "fp32"
-    ai = fl / by
-    ridge = PEAK[peak_key] / PEAK["hbm"]
-    if ai < ridge:
-        return "hbm", by, None
-    return "mfma", fl, peak_key
+    N = eng.N
+    fl = by = 0.0
+    bins = 0
+    for cl in eng.chroms:
+        big = cl.s > 128
+        if (shape == 5) != big:
+            continue
+        for tl in cl.tiles:
+            w, s = tl.width, cl.s
+            fl += tl.rows * 2.0 * s * w * N
+            by += tl.rows * float(xe) * (w * N + s * w + s * N)
+            bins += tl.rows
+    ridge = PEAK["bf16" if eng.dtype == "bf16" else "fp32"] / PEAK["hbm"]
+    bound = "hbm" if fl / by < ridge else "mfma"
+    return bound, by, fl, "%d bins, %d cells" % (bins, N)
+
+
+def traffic_from_profiles(kernel_tag):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), when present."""
+    path = os.path.join(ROOT, "profiles", "that_bf16_c2_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch[%s]:" % kernel_tag):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def ops_shape(shape):
+    from fasthigashi import ops
+    return ops.SHAPES[shape]
 
 
 def build(args):
@@ -186,8 +193,11 @@ def main():
     ap.add_argument("--cells", type=int, default=0, help="override the cell count (0: the workload's)")
     ap.add_argument("--paths", default="fp32,bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cells", type=int, default=1000)
+    ap.add_argument("--cpu-cells", type=int, default=5000)
     ap.add_argument("--cpu-chrom", type=int, default=-2, help="-2: the median-size chromosome")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="before the sweeps, copy this many MiB once with fh_copy2d (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
     args.paths = [p for p in args.paths.split(",") if p]
     if args.gpus != 1:
@@ -199,6 +209,17 @@ def main():
     import torch
     torch.cuda.init()
     wl, coos, tiles, sizes, n_i, init, engines = build(args)
+    if args.calib_copy_mib > 0:
+        from fasthigashi import ops, _lib as L
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        L.call("fh_copy2d", src.data_ptr(), L.F32, n, 1, dst.data_ptr(), L.F32, n, 1, n, 1.0, ops.stream_handle())
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written by copy2d_kernel" % (args.calib_copy_mib,
+                                                                               args.calib_copy_mib))
+        del src, dst
     res = {}
     for dt in args.paths:
         el, phases = time_path(engines[dt], args.steps, args.warmup)
@@ -212,17 +233,23 @@ def main():
     head = "bf16" if "bf16" in res else args.paths[0]
     eng = engines[head]
     phases = res[head]["phases"]
-    polar_iters = eng.pb.iters.cpu().numpy()
-    dominant = max((p for p in phases if phase_work(eng, p, polar_iters)[0] is not None), key=lambda p: phases[p])
-    bound, amount, peak_key = phase_work(eng, dominant, polar_iters)
+    # dominant single launch of the timed sweeps (per-launch HIP events on the launch stream)
+    cands = [p for p in phases if launch_work(eng, p) is not None]
+    dominant = max(cands, key=lambda p: phases[p])
+    bound, nbytes, nflops, what = launch_work(eng, dominant)
     sec = phases[dominant] / 1000.0
+    shape = int(dominant.split(":")[1])
+    kname = "gemm_mfma_kernel, %s operands, %dx%d tiles" % (head, *ops_shape(shape))
     if bound == "hbm":
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": amount / sec / 1e9, "peak": PEAK["hbm"] / 1e9,
-                "unit": "GB/s", "frac": amount / sec / PEAK["hbm"], "traffic": None}
+        roof = {"bound": "hbm", "achieved": nbytes / sec / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
+                "frac": nbytes / sec / PEAK["hbm"], "traffic": traffic_from_profiles(dominant)}
     else:
-        roof = {"kernel": dominant, "bound": "mfma", "achieved": amount / sec / 1e12,
-                "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s", "frac": amount / sec / PEAK[peak_key],
-                "traffic": None, "precision": peak_key}
+        pk = PEAK["bf16" if head == "bf16" else "fp32"]
+        roof = {"bound": "mfma", "achieved": nflops / sec / 1e12, "peak": pk / 1e12, "unit": "TFLOP/s",
+                "frac": nflops / sec / pk, "traffic": traffic_from_profiles(dominant)}
+    roof.update({"kernel": "T_hat = P^T X (%s): %s" % (dominant, kname), "launch_ms": phases[dominant],
+                 "algorithmic_bytes": nbytes, "algorithmic_flops": nflops, "work": what,
+                 "mfma_frac": nflops / sec / PEAK["bf16" if head == "bf16" else "fp32"]})
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()

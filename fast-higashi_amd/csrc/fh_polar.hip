@@ -233,8 +233,9 @@ __global__ __launch_bounds__(NT) void ns_final_kernel(const FhNsGroup* __restric
 
 // ---------------------------------------------------------------------------
 // Batched SPD inverse  Ginv = (G + eps I)^{-1}  (G symmetric s x s FP64).
-// One workgroup per system: Cholesky with the packed lower factor in LDS, then
-// every thread solves L L^T x = e_j for its own columns j, in place in the output.
+// s <= 160: register Gauss-Jordan (spd_gj_kernel below).  160 < s <= 256: one workgroup per
+// system, Cholesky with the packed lower factor in LDS, then every thread solves
+// L L^T x = e_j for its own columns j, in place in the output.
 // ---------------------------------------------------------------------------
 constexpr int CHOL_MAX = 256;
 constexpr int CHOL_LDS = 160 * 1024 - 4096;   // bytes available for the packed factor
@@ -248,6 +249,7 @@ __global__ __launch_bounds__(NT) void spd_inverse_kernel(const FhSpdSys* __restr
   extern __shared__ double Lp[];
   const FhSpdSys q = sys[blockIdx.x];
   const int64_t s = q.s;
+  if (s <= 160) return;                  // the register Gauss-Jordan kernels take these
   const int tid = threadIdx.x;
   for (int64_t i = tid; i < s; i += NT)
     for (int64_t j = 0; j <= i; ++j) Lp[pk(i, j)] = q.G[i * q.ldg + j] + (i == j ? q.eps : 0.);
@@ -588,40 +590,15 @@ template <> struct GjCfg<2> { static constexpr int RG = 16, CG = 16, SMAX = 64; 
 template <> struct GjCfg<4> { static constexpr int RG = 16, CG = 32, SMAX = 128; };
 template <> struct GjCfg<5> { static constexpr int RG = 16, CG = 32, SMAX = 160; };
 
-// One Newton step of every matrix of class WT that has not converged: H^{-1} by Gauss-Jordan,
-// the scaling, W (into Y1, or straight into Z1 as K at step 0) and the new H.
-//
-// Element (i, j) lives in a[i / RG][j / CG] of thread (i % RG, j % CG); the owner of a pivot
-// row / column follows from uniform arithmetic, so the only divergence is the owner's lane
-// mask.  Pivots are taken one at a time: the first step inverts G / ||G||_F + shift, whose
-// condition number reaches 1 / shift, and sequential elimination of an SPD matrix is
-// backward stable where an explicit inverse of a pivot block is not.  Rows/columns >= s are
-// identity padding.
-template <int WT>
-__global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void newton_gj_kernel(
-    const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin,
-    int* __restrict__ state, int step, double tol) {
-  constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX, NTH = RG * CG;
+// In-place Gauss-Jordan inverse of an SPD s x s matrix (s <= SMAX) held by the workgroup's
+// RG x CG threads: element (i, j) in a[i / RG][j / CG] of thread (i % RG, j % CG), padding
+// rows / columns set to the identity.  One barrier per pivot; the owner of a pivot row /
+// column follows from uniform arithmetic, so the only divergence is the owner's lane mask.
+template <int RG, int CG, int SMAX>
+__device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int s, double (&rowb)[2][SMAX],
+                                          double (&colb)[2][SMAX]) {
   constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
-  static_assert(RPT * RG == SMAX && CPT * CG == SMAX, "register grid must tile the matrix");
-  __shared__ double rowb[2][SMAX], colb[2][SMAX];
-  __shared__ double red[2 * (NTH / 64)];
-  const MatRef m = find_mat(groups, n_groups, mat_begin, blockIdx.x);
-  const int s = (int)m.g.s, ld = (int)m.g.ld;
-  if (ns_class(s) != WT) return;
-  const int st = state[blockIdx.x];
-  if (st & kNtDone) return;
-  double* const Hk = m.g.Z0 + m.b * m.g.stride;
-  double* const Wp = (step == 0 ? m.g.Z1 : m.g.Y1) + m.b * m.g.stride;
   const int tid = threadIdx.x, ty = tid / CG, tx = tid % CG;
-  double a[RPT][CPT];
-#pragma unroll
-  for (int r = 0; r < RPT; ++r)
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-      const int i = ty + RG * r, j = tx + CG * q;
-      a[r][q] = (i < s && j < s) ? Hk[i * ld + j] : (i == j ? 1. : 0.);
-    }
 #pragma unroll 1
   for (int k = 0; k < s; ++k) {
     const int bf = k & 1;
@@ -673,6 +650,43 @@ __global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void newton_gj_kerne
         }
     }
   }
+}
+
+// One Newton step of every matrix of class WT that has not converged: H^{-1} by Gauss-Jordan,
+// the scaling, W (into Y1, or straight into Z1 as K at step 0) and the new H.
+//
+// Element (i, j) lives in a[i / RG][j / CG] of thread (i % RG, j % CG); the owner of a pivot
+// row / column follows from uniform arithmetic, so the only divergence is the owner's lane
+// mask.  Pivots are taken one at a time: the first step inverts G / ||G||_F + shift, whose
+// condition number reaches 1 / shift, and sequential elimination of an SPD matrix is
+// backward stable where an explicit inverse of a pivot block is not.  Rows/columns >= s are
+// identity padding.
+template <int WT>
+__global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void newton_gj_kernel(
+    const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin,
+    int* __restrict__ state, int step, double tol) {
+  constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX, NTH = RG * CG;
+  constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
+  static_assert(RPT * RG == SMAX && CPT * CG == SMAX, "register grid must tile the matrix");
+  __shared__ double rowb[2][SMAX], colb[2][SMAX];
+  __shared__ double red[2 * (NTH / 64)];
+  const MatRef m = find_mat(groups, n_groups, mat_begin, blockIdx.x);
+  const int s = (int)m.g.s, ld = (int)m.g.ld;
+  if (ns_class(s) != WT) return;
+  const int st = state[blockIdx.x];
+  if (st & kNtDone) return;
+  double* const Hk = m.g.Z0 + m.b * m.g.stride;
+  double* const Wp = (step == 0 ? m.g.Z1 : m.g.Y1) + m.b * m.g.stride;
+  const int tid = threadIdx.x, ty = tid / CG, tx = tid % CG;
+  double a[RPT][CPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int i = ty + RG * r, j = tx + CG * q;
+      a[r][q] = (i < s && j < s) ? Hk[i * ld + j] : (i == j ? 1. : 0.);
+    }
+  gj_invert<RG, CG, SMAX>(a, s, rowb, colb);
   // scaling from the traces of H and H^{-1}
   double th = 0., ti = 0.;
 #pragma unroll
@@ -714,6 +728,36 @@ __global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void newton_gj_kerne
     if (dsq < tol) ns |= kNtDone;
     state[blockIdx.x] = ns;
   }
+}
+
+// out = (G + eps I)^{-1} for every system of class WT (s <= 160): Gauss-Jordan in registers.
+// Replaces torch.linalg.solve(A + 1e-10 I, B) of update_factor (parafac_integrative.py:12-15)
+// by an explicit inverse applied as a GEMM.
+template <int WT>
+__global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void spd_gj_kernel(const FhSpdSys* __restrict__ sys) {
+  constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX;
+  constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
+  __shared__ double rowb[2][SMAX], colb[2][SMAX];
+  const FhSpdSys q = sys[blockIdx.x];
+  const int s = (int)q.s;
+  if (ns_class(s) != WT) return;
+  const int tid = threadIdx.x, ty = tid / CG, tx = tid % CG;
+  double a[RPT][CPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int i = ty + RG * r, j = tx + CG * c;
+      a[r][c] = (i < s && j < s) ? q.G[i * q.ldg + j] + (i == j ? q.eps : 0.) : (i == j ? 1. : 0.);
+    }
+  gj_invert<RG, CG, SMAX>(a, s, rowb, colb);
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int i = ty + RG * r, j = tx + CG * c;
+      if (i < s && j < s) q.out[i * q.ldo + j] = a[r][c];
+    }
 }
 
 // K <- K W for the matrices stepped in launch `step` (>= 1): K in Z1 (odd step) / T (even step)
@@ -890,7 +934,11 @@ extern "C" int fh_spd_inverse(const FhSpdSys* sys_dev, int n, int64_t max_s, int
   FH_REQUIRE(max_s >= 1 && max_s <= CHOL_MAX, "spd_inverse: s=%lld out of range (max %d)", (long long)max_s, CHOL_MAX);
   size_t lds = (size_t)(max_s * (max_s + 1) / 2) * sizeof(double);
   FH_REQUIRE(lds <= (size_t)CHOL_LDS, "spd_inverse: s=%lld exceeds LDS", (long long)max_s);
-  hipLaunchKernelGGL(spd_inverse_kernel, dim3(n), dim3(NT), lds, stream, sys_dev, status_dev);
+  const dim3 grid(n);
+  if (max_s > 128) hipLaunchKernelGGL(spd_gj_kernel<5>, grid, dim3(512), 0, stream, sys_dev);
+  if (max_s > 64) hipLaunchKernelGGL(spd_gj_kernel<4>, grid, dim3(512), 0, stream, sys_dev);
+  hipLaunchKernelGGL(spd_gj_kernel<2>, grid, dim3(256), 0, stream, sys_dev);
+  if (max_s > 160) hipLaunchKernelGGL(spd_inverse_kernel, grid, dim3(NT), lds, stream, sys_dev, status_dev);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }

@@ -584,9 +584,10 @@ class SweepEngine:
         L.call("fh_segment_sum", self.pb.nuc.data_ptr(), self.seg.data_ptr(), self.K, self.xU.data_ptr(), 0, st)
         if self.dtype == "bf16":
             self.g_pt.run(stream)
-        # pass 2
-        mk("that")
-        self.g_that.run(stream)
+        # pass 2 (one marker per tile-shape launch, so each launch is timed on its own)
+        for g in self.g_that.groups:
+            mk("that:%d" % g.shape)
+            g.run(stream)
         mk("krp")
         for c, cl in enumerate(self.chroms):
             L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,

@@ -135,7 +135,8 @@ int fh_polar_newton(const FhNsGroup* groups_dev, int n_groups, const int64_t* ma
                     int max_iter, double tol, double* nuc_dev, int* iters_dev, fh_stream_t stream);
 
 /* ------------------------------------------------------------------ SPD inverse (FP64)
- * out = (G + eps I)^{-1} by Cholesky in LDS, one workgroup per system (s <= 256).
+ * out = (G + eps I)^{-1}, one workgroup per system: Gauss-Jordan in registers for s <= 160,
+ * Cholesky in LDS for 160 < s <= 256.
  * Replaces torch.linalg.solve in update_factor, parafac_integrative.py:12-15. */
 typedef struct FhSpdSys {
   const double* G;

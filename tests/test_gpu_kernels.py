@@ -164,16 +164,16 @@ def test_spd_inverse(cuda):
     from fasthigashi import ops, _lib as L
     rng = np.random.default_rng(2)
     mats, outs, sys = [], [], []
-    for s in (5, 64, 150):
+    for s in (5, 64, 150, 190):
         X = rng.standard_normal((s + 10, s))
         G = torch.from_numpy(X.T @ X).to(cuda)
         O = torch.zeros(s, s, dtype=torch.float64, device=cuda)
         q = L.SpdSys()
         q.G, q.out, q.s, q.ldg, q.ldo, q.eps = G.data_ptr(), O.data_ptr(), s, s, s, 1e-10
         mats.append(G); outs.append(O); sys.append(q)
-    tab = ops._upload((L.SpdSys * 3)(*sys), cuda)
+    tab = ops._upload((L.SpdSys * 4)(*sys), cuda)
     status = torch.zeros(1, dtype=torch.int32, device=cuda)
-    L.call("fh_spd_inverse", tab.data_ptr(), 3, 150, status.data_ptr(), ops.stream_handle())
+    L.call("fh_spd_inverse", tab.data_ptr(), 4, 190, status.data_ptr(), ops.stream_handle())
     for G, O in zip(mats, outs):
         Gn = G.cpu().numpy() + 1e-10 * np.eye(G.shape[0])
         ref = np.linalg.inv(Gn)
