@@ -31,20 +31,25 @@ def log(*a):
 def launch_work(eng, phase):
     """Algorithmic work of one single-launch phase: (bound, bytes, flops, description).
 
-    'that:<shape>' is one grouped launch of T_hat_i = P_i^T X_i over every bin whose
-    chromosome uses that tile shape: per bin 2 s w N flops; bytes = X_i (w x N) +
-    P_i (s x w) read once, T_hat_i (s x N) written once, at the path's element size.
-    Padding (cells up to a multiple of 64, positions up to a multiple of 8) is not counted."""
+    'that:b<bm>' (streaming bf16 kernel, tile height bm) and 'that:<shape>' (tiled kernel)
+    are one grouped launch of T_hat_i = P_i^T X_i over every bin whose chromosome uses that
+    tile: per bin 2 s w N flops; bytes = X_i (w x N) + P_i (s x w) read once, T_hat_i (s x N)
+    written once, at the path's element size.  Padding (cells up to a multiple of 64,
+    positions up to a multiple of 64) is not counted."""
     if not phase.startswith("that:"):
         return None
-    shape = int(phase.split(":")[1])
+    from fasthigashi import ops
+    tag = phase.split(":")[1]
+    if tag.startswith("b"):
+        member = lambda s: ops.stream_bm(s) == int(tag[1:])
+    else:
+        member = lambda s: (int(tag) == 5) == (s > 128)
     xe = 2 if eng.dtype == "bf16" else 4
     N = eng.N
     fl = by = 0.0
     bins = 0
     for cl in eng.chroms:
-        big = cl.s > 128
-        if (shape == 5) != big:
+        if not member(cl.s):
             continue
         for tl in cl.tiles:
             w, s = tl.width, cl.s
@@ -54,6 +59,14 @@ def launch_work(eng, phase):
     ridge = PEAK["bf16" if eng.dtype == "bf16" else "fp32"] / PEAK["hbm"]
     bound = "hbm" if fl / by < ridge else "mfma"
     return bound, by, fl, "%d bins, %d cells" % (bins, N)
+
+
+def kernel_name(phase, head):
+    from fasthigashi import ops
+    tag = phase.split(":")[1]
+    if tag.startswith("b"):
+        return "gemm_stream_kernel, bf16 operands, %dx128 tiles, global->LDS ring" % int(tag[1:])
+    return "gemm_mfma_kernel, %s operands, %dx%d tiles" % (head, *ops.SHAPES[int(tag)])
 
 
 def traffic_from_profiles(kernel_tag):
@@ -66,11 +79,6 @@ def traffic_from_profiles(kernel_tag):
         if line.startswith("traffic_bytes_per_launch[%s]:" % kernel_tag):
             return float(line.split(":", 1)[1])
     return None
-
-
-def ops_shape(shape):
-    from fasthigashi import ops
-    return ops.SHAPES[shape]
 
 
 def build(args):
@@ -238,8 +246,7 @@ def main():
     dominant = max(cands, key=lambda p: phases[p])
     bound, nbytes, nflops, what = launch_work(eng, dominant)
     sec = phases[dominant] / 1000.0
-    shape = int(dominant.split(":")[1])
-    kname = "gemm_mfma_kernel, %s operands, %dx%d tiles" % (head, *ops_shape(shape))
+    kname = kernel_name(dominant, head)
     if bound == "hbm":
         roof = {"bound": "hbm", "achieved": nbytes / sec / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
                 "frac": nbytes / sec / PEAK["hbm"], "traffic": traffic_from_profiles(dominant)}

@@ -1,0 +1,384 @@
+// Streaming BF16 GEMM for k-contiguous operands: persistent workgroups, a ring of LDS
+// stages filled by direct global->LDS loads (global_load_lds_dwordx4), MFMA 16x16x32 bf16,
+// fp32 accumulate.
+//
+// The X-streaming contraction of the sweep (T_hat_i = P_i^T X_i) has short k per output tile
+// and a large streamed operand; with one register-staged prefetch step the general kernel
+// leaves most of the HBM latency exposed.  Here each workgroup walks its tiles
+// (t = blockIdx.x, + gridDim.x, ...) as one flat sequence of k-steps and keeps STAGES - 1
+// steps in flight across tile boundaries; consecutive tile ids share a descriptor batch
+// (same stationary operand), so the workgroups work on a narrow window of the stationary
+// data and it stays in L2.
+//
+// LDS stage layout: BM rows of A then BN rows of B, each row one 128-byte k-chunk-row
+// (64 bf16), its eight 16-byte chunks stored at position chunk ^ (row & 7).  A 1 KB load
+// instruction of a wave fills 8 rows: lane l writes position l % 8 of row l / 8 and
+// therefore loads global chunk (l % 8) ^ (row & 7).
+//
+// The loads and the stage waits are written out (inline asm), so the compiler tracks none
+// of them and adds no waits of its own; the loop body exists once, with the stage picked at
+// run time, which keeps the kernel small enough that its once-per-tile output path stays in
+// the instruction cache.
+//
+// Contract (checked by the host): both operands k-contiguous, leading dims and batch strides
+// multiples of 8 elements, 16-byte aligned bases, and every row readable and zero from K up
+// to the next multiple of 64 (k-steps load whole 64-element chunks).  flags: 0 or
+// FH_GEMM_OUT_BF16; beta and diag must be 0.
+#include "fh_common.h"
+
+namespace {
+
+constexpr int SNT = 256;   // 4 waves, 2 x 2 over the output tile
+constexpr int STAGES = 4;
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+typedef __attribute__((address_space(1))) uint16_t gu16;
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) u32x4 gu32x4;
+
+__device__ __attribute__((aligned(16))) uint32_t g_zero16[4];
+
+// The descriptor table is read through the constant address space: it is never written while
+// the kernel runs, and constant loads stay scalar even after the kernel's own global stores.
+typedef const __attribute__((address_space(4))) FhGemmDesc CDesc;
+
+// A cursor's tile ids only grow, so its descriptor index only moves forward: one scalar load
+// per tile instead of a search.
+__device__ __forceinline__ int advance_desc(CDesc* d, int n, int di, int64_t t) {
+  while (di + 1 < n && d[di + 1].tile_begin <= t) ++di;
+  return di;
+}
+
+// batch and first row / column of tile t of descriptor d (tile counts fit 32 bits, checked
+// by the host)
+struct TilePos {
+  int64_t b, m0, n0;
+};
+__device__ __forceinline__ TilePos tile_pos(CDesc& d, int64_t t, int BM, int BN) {
+  uint32_t local = (uint32_t)(t - d.tile_begin);
+  const uint32_t tiles_n = (uint32_t)d.tiles_n;
+  const uint32_t per_batch = (uint32_t)d.tiles_m * tiles_n;
+  const uint32_t b = local / per_batch;
+  local -= b * per_batch;
+  const uint32_t tm = local / tiles_n;
+  return TilePos{(int64_t)b, (int64_t)tm * BM, (int64_t)(local - tm * tiles_n) * BN};
+}
+
+// what the output of one tile needs
+struct StoreRef {
+  void* C;           // element (m0, n0) of the output (batch applied)
+  int64_t sCm, sCn, rows, cols;
+  float alpha;
+  int nk;
+  bool live, bf16, vec;   // vec: C n-contiguous with rows 16-byte aligned
+};
+
+__device__ __forceinline__ StoreRef store_ref(CDesc* descs, int n_desc, int& di, int64_t t, int64_t total, int BM,
+                                              int BN) {
+  StoreRef r{};
+  if (t >= total) {
+    r.live = false;
+    r.nk = 1;
+    return r;
+  }
+  di = advance_desc(descs, n_desc, di, t);
+  CDesc& d = descs[di];
+  const TilePos p = tile_pos(d, t, BM, BN);
+  r.bf16 = (d.flags & FH_GEMM_OUT_BF16) != 0;
+  const int64_t off = p.b * d.sCb + p.m0 * d.sCm + p.n0 * d.sCn;
+  r.C = r.bf16 ? (void*)((uint16_t*)d.C + off) : (void*)((float*)d.C + off);
+  r.sCm = d.sCm;
+  r.sCn = d.sCn;
+  r.rows = d.M - p.m0;
+  r.cols = d.N - p.n0;
+  r.alpha = (float)d.alpha;
+  r.nk = (int)((d.K + 63) / 64);
+  r.live = true;
+  const int al = r.bf16 ? 7 : 3;   // outputs per 16 bytes, minus one
+  r.vec = (((uintptr_t)d.C) & 15) == 0 && d.sCn == 1 && (d.sCm & al) == 0 && (d.sCb & al) == 0;
+  return r;
+}
+
+// output staging: per wave 16 rows of 128 bytes (64 bf16 or 32 fp32), padded to 144 bytes
+constexpr int kStageRow = 144;
+
+template <int WM_T, int WN_T>
+struct StreamCfg {
+  static constexpr int BM = 32 * WM_T, BN = 32 * WN_T;
+  static constexpr int DA = BM / 8, DB = BN / 8;   // 1 KB load instructions per stage
+  static_assert((DA + DB) % 4 == 0, "load instructions must split evenly over the 4 waves");
+  static constexpr int D = (DA + DB) / 4;          // per wave per stage
+  static constexpr int STAGE_BYTES = (BM + BN) * 128;
+};
+
+// What the loads of one tile need: per load instruction q of the wave, the lane's source
+// address at k = 0 (row clamped to the operand's last row: rows past M or N load a copy of
+// a valid row, whose outputs are never stored), and the byte step per 64-wide k-chunk.
+// Computed once per tile, so a k-step issues its loads with one address add each.
+template <int D>
+struct LoadRef {
+  const uint8_t* p[D];
+  uint32_t kstep;    // 128 (bytes of 64 bf16), 0 for a tile past the end
+  int nk;            // k-steps
+  bool live;
+};
+
+template <int WM_T, int WN_T>
+__device__ __forceinline__ LoadRef<StreamCfg<WM_T, WN_T>::D> load_ref(CDesc* descs, int n_desc, int& di, int64_t t,
+                                                                     int64_t total, int wave, int lane) {
+  using C = StreamCfg<WM_T, WN_T>;
+  LoadRef<C::D> r;
+  if (t >= total) {
+#pragma unroll
+    for (int q = 0; q < C::D; ++q) r.p[q] = (const uint8_t*)g_zero16;
+    r.kstep = 0;
+    r.nk = 1;
+    r.live = false;
+    return r;
+  }
+  di = advance_desc(descs, n_desc, di, t);
+  CDesc& d = descs[di];
+  const TilePos pos = tile_pos(d, t, C::BM, C::BN);
+  const uint8_t* baseA = (const uint8_t*)((const __bf16*)d.A + pos.b * d.sAb + pos.m0 * d.sAm);
+  const uint8_t* baseB = (const uint8_t*)((const __bf16*)d.B + pos.b * d.sBb + pos.n0 * d.sBn);
+  const int64_t rows_a = d.M - pos.m0, rows_b = d.N - pos.n0;
+#pragma unroll
+  for (int q = 0; q < C::D; ++q) {
+    const int i = wave + 4 * q;                     // load instruction index in the stage
+    const bool isA = i < C::DA;                     // uniform
+    const int blk = isA ? i : i - C::DA;
+    const int row = blk * 8 + (lane >> 3);
+    const int c = (lane & 7) ^ (row & 7);
+    const int64_t rr = min((int64_t)row, (isA ? rows_a : rows_b) - 1);
+    r.p[q] = (isA ? baseA : baseB) + 2 * rr * (isA ? d.sAm : d.sBn) + 16 * c;
+  }
+  r.kstep = 128;
+  r.nk = (int)((d.K + 63) / 64);
+  r.live = true;
+  return r;
+}
+
+// One 1 KB global->LDS load of the wave: lane l copies 16 bytes from src to LDS address
+// lds + 16 l (M0 holds the LDS base; one wait state between the M0 write and its use).
+__device__ __forceinline__ void dma16(const void* src, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(src), "s"(__builtin_amdgcn_readfirstlane(lds))
+               : "memory", "m0");
+}
+
+// issue the loads of k-step kk of tile r into the stage at LDS address st
+template <int D>
+__device__ __forceinline__ void issue_stage(const LoadRef<D>& r, int kk, uint32_t st, int wave) {
+  const uint32_t kb = (uint32_t)kk * r.kstep;
+#pragma unroll
+  for (int q = 0; q < D; ++q) dma16(r.p[q] + kb, st + (wave + 4 * q) * 1024);
+}
+
+// s_waitcnt vmcnt(n) then the workgroup barrier; a compiler memory fence, so no LDS access
+// moves across it
+template <int N>
+__device__ __forceinline__ void wait_vm_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt out of range");
+  asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" : : "n"(N) : "memory");
+}
+
+template <int WM_T, int WN_T>
+__device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[WM_T][WN_T], int wm, int wn,
+                                              int lane) {
+  using C = StreamCfg<WM_T, WN_T>;
+  const uint8_t* As = st;
+  const uint8_t* Bs = st + C::BM * 128;
+  u32x4 bf[WN_T][2];
+#pragma unroll
+  for (int j = 0; j < WN_T; ++j) {
+    const int row = wn * WN_T * 16 + j * 16 + (lane & 15);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int c = 2 * (lane >> 4) + t;
+      bf[j][t] = *(const u32x4*)(Bs + row * 128 + ((c ^ (row & 7)) << 4));
+    }
+  }
+  u32x4 af[WM_T][2];
+#pragma unroll
+  for (int i = 0; i < WM_T; ++i) {
+    const int row = wm * WM_T * 16 + i * 16 + (lane & 15);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int c = 2 * (lane >> 4) + t;
+      af[i][t] = *(const u32x4*)(As + row * 128 + ((c ^ (row & 7)) << 4));
+    }
+  }
+  // B is the first MFMA operand, so acc[i][j] holds C^T blocks: lane l has
+  // C[m = 16 i + l % 16][n = 16 j + 4 (l / 16) + r], r = 0..3 (four consecutive n).
+  // The k-halves t are the outer loop, so consecutive MFMAs use independent accumulators.
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+      for (int j = 0; j < WN_T; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bf[j][t]),
+                                                            __builtin_bit_cast(bf16x8, af[i][t]), acc[i][j], 0, 0, 0);
+}
+
+// Output of one tile through the wave's staging rows, for outputs of ESZ bytes: per 16-row
+// block and per 128-byte column group (64 bf16 or 32 fp32 outputs), each lane writes its
+// four consecutive n at row l % 16, then reads back 16 bytes of row h * 8 + l / 8 (chunk
+// l % 8), so one store covers 8 full 128-byte rows of C.  Rows that are not 16-byte aligned,
+// and the ragged n edge, are written element by element from the staged rows (a rolled loop).
+template <int ESZ, int WM_T, int WN_T>
+__device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[WM_T][WN_T], int wm, int wn, int lane,
+                                           uint8_t* so) {
+  static_assert(WN_T == 4, "a wave's 64 columns are 1 (bf16) or 2 (fp32) staged 128-byte groups");
+  constexpr int PER16 = 16 / ESZ;                  // outputs per 16-byte chunk
+  constexpr int GROUPS = ESZ / 2;                  // 128-byte column groups of the wave
+  constexpr int JPG = WN_T / GROUPS;               // 16-column blocks per group
+  const float alpha = sr.alpha;
+  const int64_t sCm = sr.sCm, sCn = sr.sCn, rows = sr.rows, cols = sr.cols;
+#pragma unroll
+  for (int i = 0; i < WM_T; ++i) {
+    const int64_t mb = wm * WM_T * 16 + i * 16;
+    if (mb < rows) {                               // uniform
+#pragma unroll
+      for (int g = 0; g < GROUPS; ++g) {
+#pragma unroll
+        for (int jj = 0; jj < JPG; ++jj) {
+          const floatx4 v = acc[i][g * JPG + jj] * alpha;
+          uint8_t* dst = so + (lane & 15) * kStageRow + (jj * 16 + (lane >> 4) * 4) * ESZ;
+          if (ESZ == 2) {
+            u32x2 w;
+            w.x = (uint32_t)fh_f32_to_bf16_bits(v[0]) | ((uint32_t)fh_f32_to_bf16_bits(v[1]) << 16);
+            w.y = (uint32_t)fh_f32_to_bf16_bits(v[2]) | ((uint32_t)fh_f32_to_bf16_bits(v[3]) << 16);
+            *(u32x2*)dst = w;
+          } else {
+            *(floatx4*)dst = v;
+          }
+        }
+        const int64_t nb = wn * WN_T * 16 + g * (JPG * 16);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int row = h * 8 + (lane >> 3), ch = lane & 7;
+          const int64_t m = mb + row, n = nb + ch * PER16;
+          const uint8_t* srcrow = so + row * kStageRow + ch * 16;
+          if (sr.vec && n + PER16 <= cols) {
+            const u32x4 v = *(const u32x4*)srcrow;
+            if (m < rows) {
+              if (ESZ == 2) *(gu32x4*)((gu16*)sr.C + m * sCm + n) = v;
+              else *(gu32x4*)((gfloat*)sr.C + m * sCm + n) = v;
+            }
+          } else if (m < rows) {
+#pragma unroll 1
+            for (int e = 0; e < PER16 && n + e < cols; ++e) {
+              if (ESZ == 2) ((gu16*)sr.C)[m * sCm + (n + e) * sCn] = *(const uint16_t*)(srcrow + 2 * e);
+              else ((gfloat*)sr.C)[m * sCm + (n + e) * sCn] = *(const float*)(srcrow + 4 * e);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// One barrier per k-step: at step k every wave has waited for its loads of stage k % 4 and
+// finished step k - 1, so the stage read at step k - 1 is refilled with step k + 3 (the load
+// cursor runs STAGES - 1 steps ahead, through zero blocks past the last tile).  The output
+// stores count in vmcnt too; the stage wait stays exact because a wave's loads retire in
+// issue order, so vmcnt <= n leaves at most its n youngest loads outstanding.
+template <int WM_T, int WN_T>
+__global__ __launch_bounds__(SNT) void gemm_stream_kernel(const FhGemmDesc* __restrict__ descs_arg, int n_desc,
+                                                          int64_t total_tiles) {
+  using C = StreamCfg<WM_T, WN_T>;
+  CDesc* descs = (CDesc*)descs_arg;
+  __shared__ __attribute__((aligned(1024))) uint8_t ring[STAGES * C::STAGE_BYTES];
+  __shared__ __attribute__((aligned(16))) uint8_t SO[4][16 * kStageRow];
+  const uint32_t ring_lds = (uint32_t)(uintptr_t)(lds_ptr_t)ring;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  const int64_t stride = gridDim.x;
+
+  // load cursor
+  int64_t lt = blockIdx.x;
+  int dl = 0;
+  LoadRef<C::D> lr = load_ref<WM_T, WN_T>(descs, n_desc, dl, lt, total_tiles, wave, lane);
+  int lk = 0;
+  // compute cursor
+  int64_t ct = blockIdx.x;
+  int dc = 0;
+  StoreRef sr = store_ref(descs, n_desc, dc, ct, total_tiles, C::BM, C::BN);
+  int ck = 0;
+
+  floatx4 acc[WM_T][WN_T];
+#pragma unroll
+  for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+    for (int j = 0; j < WN_T; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  auto issue = [&](int stage) {
+    issue_stage(lr, lk, ring_lds + stage * C::STAGE_BYTES, wave);
+    if (++lk >= lr.nk) {
+      lk = 0;
+      lt += stride;
+      lr = load_ref<WM_T, WN_T>(descs, n_desc, dl, lt, total_tiles, wave, lane);
+    }
+  };
+  issue(0);
+  issue(1);
+  issue(2);
+  for (int k = 0; sr.live; ++k) {
+    wait_vm_barrier<C::D * (STAGES - 2)>();
+    issue((k + STAGES - 1) % STAGES);
+    compute_stage<WM_T, WN_T>(ring + (k % STAGES) * C::STAGE_BYTES, acc, wm, wn, lane);
+    if (++ck >= sr.nk) {
+      if (sr.bf16) store_tile<2>(sr, acc, wm, wn, lane, SO[wave]);
+      else store_tile<4>(sr, acc, wm, wn, lane, SO[wave]);
+#pragma unroll
+      for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+        for (int j = 0; j < WN_T; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+      ck = 0;
+      ct += stride;
+      sr = store_ref(descs, n_desc, dc, ct, total_tiles, C::BM, C::BN);
+    }
+  }
+  asm volatile("s_waitcnt vmcnt(0)" : : : "memory");   // no load may still target LDS at exit
+}
+
+}  // namespace
+
+// Streaming bf16 GEMM over a descriptor table prepared by fh_gemm_prepare(descs, n, bm, bn).
+// bm in {64, 96, 128, 160}, bn = 128.  grid: persistent workgroups (0 = one per CU).
+extern "C" int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
+                                     int grid, hipStream_t stream) {
+  if (total_tiles <= 0) return FH_OK;
+  FH_REQUIRE(descs_dev != nullptr && n_desc > 0, "gemm_stream: descriptor table required");
+  FH_REQUIRE(total_tiles < ((int64_t)1 << 31), "gemm_stream: %lld tiles exceed the 32-bit tile index",
+             (long long)total_tiles);
+  FH_REQUIRE(bn == 128, "gemm_stream: bn must be 128 (got %d)", bn);
+  if (grid <= 0) {
+    int dev = 0, cus = 0;
+    FH_CHECK_HIP(hipGetDevice(&dev));
+    FH_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    grid = cus;
+  }
+  if ((int64_t)grid > total_tiles) grid = (int)total_tiles;
+  switch (bm) {
+    case 64:
+      hipLaunchKernelGGL((gemm_stream_kernel<2, 4>), dim3(grid), dim3(SNT), 0, stream, descs_dev, n_desc, total_tiles);
+      break;
+    case 96:
+      hipLaunchKernelGGL((gemm_stream_kernel<3, 4>), dim3(grid), dim3(SNT), 0, stream, descs_dev, n_desc, total_tiles);
+      break;
+    case 128:
+      hipLaunchKernelGGL((gemm_stream_kernel<4, 4>), dim3(grid), dim3(SNT), 0, stream, descs_dev, n_desc, total_tiles);
+      break;
+    case 160:
+      hipLaunchKernelGGL((gemm_stream_kernel<5, 4>), dim3(grid), dim3(SNT), 0, stream, descs_dev, n_desc, total_tiles);
+      break;
+    default:
+      fh_set_error("gemm_stream: unsupported tile height %d (64, 96, 128 or 160)", bm);
+      return FH_ERR_ARG;
+  }
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}

@@ -82,6 +82,7 @@ _SIGS = {
     "fh_gemm_shape_dims": (None, [c_int, POINTER(c_int), POINTER(c_int)]),
     "fh_gemm_launch": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int64, c_void_p]),
     "fh_gemm64_launch": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int64, c_void_p]),
+    "fh_gemm_stream_launch": (c_int, [c_int, c_int, c_void_p, c_int, c_int64, c_int, c_void_p]),
     "fh_ns_prepare": (c_int64, [c_void_p, c_int]),
     "fh_ns_run": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_int64, c_int, c_double, c_void_p,
                           POINTER(c_int), c_void_p]),

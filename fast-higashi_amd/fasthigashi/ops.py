@@ -139,6 +139,45 @@ class GemmGroup:
                    None, self.total, st)
 
 
+STREAM_BM = (64, 96, 128, 160)
+
+
+def stream_bm(M: int) -> int:
+    """smallest streaming tile height covering M rows (largest for M > 160)."""
+    for bm in STREAM_BM:
+        if M <= bm:
+            return bm
+    return STREAM_BM[-1]
+
+
+class StreamGemmGroup:
+    """Grouped launch of the streaming bf16 GEMM (both operands k-contiguous, persistent
+    workgroups, 4-deep global->LDS ring); tiles bm x 128."""
+
+    def __init__(self, descs: Sequence[L.GemmDesc], bm: Optional[int] = None, grid: int = 0, device="cuda"):
+        assert len(descs) > 0
+        for d in descs:
+            assert d.sAk == 1 and d.sBk == 1, "streaming GEMM needs k-contiguous A and B"
+            # k-steps load whole 64-element chunks: rows readable (and zero) up to K rounded to 64
+            for ld, bs, p, nm in ((d.sAm, d.sAb, d.A, "A"), (d.sBn, d.sBb, d.B, "B")):
+                assert ld % 8 == 0 and ld >= -(-d.K // 64) * 64, "%s leading dim %d (K %d)" % (nm, ld, d.K)
+                assert d.batch == 1 or bs % 8 == 0, "%s batch stride %d not a multiple of 8" % (nm, bs)
+                assert p % 16 == 0, "%s pointer not 16-byte aligned" % nm
+            assert d.beta == 0.0 and d.diag == 0.0 and (d.flags & ~L.GEMM_OUT_BF16) == 0, \
+                "streaming GEMM: flags 0 or OUT_BF16, no beta/diag"
+        self.bm = bm if bm is not None else stream_bm(max(d.M for d in descs))
+        self.grid = grid
+        arr = (L.GemmDesc * len(descs))(*descs)
+        self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), self.bm, 128)
+        self.n = len(descs)
+        self.descs = list(arr)
+        self.table = _upload(arr, device)
+
+    def run(self, stream=None):
+        L.call("fh_gemm_stream_launch", self.bm, 128, self.table.data_ptr(), self.n, self.total, self.grid,
+               stream_handle(stream))
+
+
 class Gemm64Group:
     """Grouped launch of the FP64 MFMA GEMM (operands fp32 or fp64, output fp32 or fp64)."""
 

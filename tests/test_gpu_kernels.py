@@ -263,3 +263,64 @@ def test_polar_newton_matches_eigh(cuda, log_kappa):
         assert np.allclose(nuc, ref_nuc, rtol=1e-9)
     newton = np.concatenate([np.full(c, s <= 160) for s, c in zip(sizes, counts)])
     assert (its[newton] <= 14).all(), its
+
+
+def _stream_case(cuda, gen, M, N, K, batch, out16):
+    """operands of T_hat = P^T X in the streaming layout: A (batch, sp, wp) and B (batch, N, wp),
+    both k-contiguous with the k-padding (to a multiple of 64) zero; C (batch, sp, N)."""
+    from fasthigashi import ops, _lib as L
+    wp = -(-K // 64) * 64
+    sp = -(-M // 4) * 4
+    A = torch.zeros(batch, sp, wp)
+    B = torch.zeros(batch, N, wp)
+    A[:, :M, :K] = torch.randn(batch, M, K, generator=gen)
+    B[:, :, :K] = torch.randn(batch, N, K, generator=gen)
+    A = A.to(torch.bfloat16).to(cuda)
+    B = B.to(torch.bfloat16).to(cuda)
+    C = torch.full((batch, sp, N), 7.0, dtype=torch.bfloat16 if out16 else torch.float32, device=cuda)
+    d = ops.make_desc(A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K, wp, 1, 1, wp, N, 1, batch,
+                      sp * wp, N * wp, sp * N, alpha=0.5, flags=L.GEMM_OUT_BF16 if out16 else 0)
+    ref = 0.5 * (A[:, :M, :K].double().cpu() @ B[:, :, :K].double().cpu().transpose(1, 2))
+    # A and B are returned so they stay allocated while the descriptor points at them
+    return d, C, ref, M, (A, B)
+
+
+@pytest.mark.parametrize("grid", [0, 7])
+def test_gemm_stream_matches_reference(cuda, grid):
+    """streaming bf16 GEMM over a grouped table of uneven descriptors (tile heights 64..160,
+    K below, at and above one 64-wide step, K not a multiple of 8, ragged N, batches; bf16
+    output both through the staged 16-byte path (N a multiple of 8) and per element):
+    fp32 output within fp32-accumulation rounding, bf16 output within one bf16 rounding;
+    rows of C beyond M untouched."""
+    from fasthigashi import ops
+    gen = torch.Generator().manual_seed(29)
+    cases = [(28, 300, 40, 3, False), (150, 1000, 200, 2, False), (97, 129, 64, 4, True),
+             (128, 256, 130, 1, False), (60, 77, 13, 5, True), (150, 1000, 200, 2, True),
+             (33, 264, 72, 3, True)]
+    for bm in (64, 96, 128, 160):
+        built = [_stream_case(cuda, gen, M, N, K, b, o) for (M, N, K, b, o) in cases]
+        ops.StreamGemmGroup([x[0] for x in built], bm=bm, grid=grid, device=cuda).run()
+        torch.cuda.synchronize()
+        for (d, C, ref, M, _), case in zip(built, cases):
+            got = C[:, :M].double().cpu()
+            scale = ref.abs().max().item()
+            tol = 1e-2 if case[4] else 1e-5
+            err = (got - ref).abs().max().item() / scale
+            assert err < tol, (bm, grid, case, err)
+            assert bool((C[:, M:].float() == 7.0).all()), (bm, case)
+
+
+def test_gemm_stream_matches_tiled_gemm(cuda):
+    """same descriptors through the streaming kernel and the general tiled bf16 GEMM."""
+    from fasthigashi import ops, _lib as L
+    gen = torch.Generator().manual_seed(31)
+    built = [_stream_case(cuda, gen, 150, 2000, 190, 3, False)]
+    d, C, ref, M, _ = built[0]
+    ops.StreamGemmGroup([d], device=cuda).run()
+    C2 = torch.zeros_like(C)
+    d2 = L.GemmDesc.from_buffer_copy(bytes(d))
+    d2.C = C2.data_ptr()
+    ops.GemmGroup([d2], L.BF16, shape=5).run()
+    torch.cuda.synchronize()
+    diff = (C[:, :M] - C2[:, :M]).abs().max().item() / C2.abs().max().item()
+    assert diff < 1e-5, diff
