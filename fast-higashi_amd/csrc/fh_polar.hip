@@ -592,62 +592,60 @@ template <> struct GjCfg<5> { static constexpr int RG = 16, CG = 32, SMAX = 160;
 
 // In-place Gauss-Jordan inverse of an SPD s x s matrix (s <= SMAX) held by the workgroup's
 // RG x CG threads: element (i, j) in a[i / RG][j / CG] of thread (i % RG, j % CG), padding
-// rows / columns set to the identity.  One barrier per pivot; the owner of a pivot row /
-// column follows from uniform arithmetic, so the only divergence is the owner's lane mask.
+// rows / columns set to the identity.  One barrier per pivot.
+//
+// Pivot k = kr RG + ky: the outer loop over kr is unrolled, so the owners' register slots
+// (row slot kr, column slot kq = kr RG / CG) are compile-time: publishing row / column k and
+// rewriting them after the rank-1 update a <- a - u v^T (u = column k, v = row k / p) are
+// plain masked register moves, with no selects over slots.  The pivot owner broadcasts 1 / p.
 template <int RG, int CG, int SMAX>
 __device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int s, double (&rowb)[2][SMAX],
-                                          double (&colb)[2][SMAX]) {
+                                          double (&colb)[2][SMAX], double (&ipb)[2]) {
   constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
+  static_assert(CG % RG == 0 && RG % 2 == 0, "column slot of a pivot must follow from its row slot");
   const int tid = threadIdx.x, ty = tid / CG, tx = tid % CG;
+#pragma unroll
+  for (int kr = 0; kr < RPT; ++kr) {
+    const int kq = kr * RG / CG;                     // compile-time in the unrolled loop
+    const int kend = min(RG, s - kr * RG);
 #pragma unroll 1
-  for (int k = 0; k < s; ++k) {
-    const int bf = k & 1;
-    const int kr = k / RG, ky = k - kr * RG, kq = k / CG, kx = k - kq * CG;
-    if (ty == ky) {
+    for (int ky = 0; ky < kend; ++ky) {
+      const int bf = ky & 1;
+      const int kx = kr * RG - kq * CG + ky;
+      if (ty == ky) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) rowb[bf][tx + CG * q] = a[kr][q];
+      }
+      if (tx == kx) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) colb[bf][ty + RG * r] = a[r][kq];
+        if (ty == ky) {
+          double p = a[kr][kq];
+          p = p > 1e-300 ? p : 1e-300;
+          ipb[bf] = 1. / p;
+        }
+      }
+      __syncthreads();
+      const double ip = ipb[bf];
+      double v[CPT], u[RPT];
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) v[q] = rowb[bf][tx + CG * q] * ip;
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) u[r] = colb[bf][ty + RG * r];
 #pragma unroll
       for (int r = 0; r < RPT; ++r)
-        if (r == kr) {
 #pragma unroll
-          for (int q = 0; q < CPT; ++q) rowb[bf][tx + CG * q] = a[r][q];
-        }
-    }
-    if (tx == kx) {
+        for (int q = 0; q < CPT; ++q) a[r][q] = fma(-u[r], v[q], a[r][q]);
+      // row k <- A[k, :] / p, column k <- -A[:, k] / p, (k, k) <- 1 / p
+      if (ty == ky) {
 #pragma unroll
-      for (int q = 0; q < CPT; ++q)
-        if (q == kq) {
+        for (int q = 0; q < CPT; ++q) a[kr][q] = v[q];
+      }
+      if (tx == kx) {
 #pragma unroll
-          for (int r = 0; r < RPT; ++r) colb[bf][ty + RG * r] = a[r][q];
-        }
-    }
-    __syncthreads();
-    double piv = rowb[bf][k];
-    piv = piv > 1e-300 ? piv : 1e-300;
-    const double ip = 1. / piv;
-    double v[CPT], u[RPT];
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) v[q] = rowb[bf][tx + CG * q] * ip;
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) u[r] = colb[bf][ty + RG * r];
-    // Schur update everywhere, then row k <- A[k, :] / p, column k <- -A[:, k] / p, (k, k) <- 1 / p
-#pragma unroll
-    for (int r = 0; r < RPT; ++r)
-#pragma unroll
-      for (int q = 0; q < CPT; ++q) a[r][q] = fma(-u[r], v[q], a[r][q]);
-    if (ty == ky) {
-#pragma unroll
-      for (int r = 0; r < RPT; ++r)
-        if (r == kr) {
-#pragma unroll
-          for (int q = 0; q < CPT; ++q) a[r][q] = v[q];
-        }
-    }
-    if (tx == kx) {
-#pragma unroll
-      for (int q = 0; q < CPT; ++q)
-        if (q == kq) {
-#pragma unroll
-          for (int r = 0; r < RPT; ++r) a[r][q] = (ty == ky && r == kr) ? ip : -u[r] * ip;
-        }
+        for (int r = 0; r < RPT; ++r) a[r][kq] = -u[r] * ip;
+        if (ty == ky) a[kr][kq] = ip;
+      }
     }
   }
 }
@@ -668,7 +666,7 @@ __global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void newton_gj_kerne
   constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX, NTH = RG * CG;
   constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
   static_assert(RPT * RG == SMAX && CPT * CG == SMAX, "register grid must tile the matrix");
-  __shared__ double rowb[2][SMAX], colb[2][SMAX];
+  __shared__ double rowb[2][SMAX], colb[2][SMAX], ipb[2];
   __shared__ double red[2 * (NTH / 64)];
   const MatRef m = find_mat(groups, n_groups, mat_begin, blockIdx.x);
   const int s = (int)m.g.s, ld = (int)m.g.ld;
@@ -686,7 +684,7 @@ __global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void newton_gj_kerne
       const int i = ty + RG * r, j = tx + CG * q;
       a[r][q] = (i < s && j < s) ? Hk[i * ld + j] : (i == j ? 1. : 0.);
     }
-  gj_invert<RG, CG, SMAX>(a, s, rowb, colb);
+  gj_invert<RG, CG, SMAX>(a, s, rowb, colb, ipb);
   // scaling from the traces of H and H^{-1}
   double th = 0., ti = 0.;
 #pragma unroll
@@ -737,7 +735,7 @@ template <int WT>
 __global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void spd_gj_kernel(const FhSpdSys* __restrict__ sys) {
   constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX;
   constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
-  __shared__ double rowb[2][SMAX], colb[2][SMAX];
+  __shared__ double rowb[2][SMAX], colb[2][SMAX], ipb[2];
   const FhSpdSys q = sys[blockIdx.x];
   const int s = (int)q.s;
   if (ns_class(s) != WT) return;
@@ -750,7 +748,7 @@ __global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void spd_gj_kernel(c
       const int i = ty + RG * r, j = tx + CG * c;
       a[r][c] = (i < s && j < s) ? q.G[i * q.ldg + j] + (i == j ? q.eps : 0.) : (i == j ? 1. : 0.);
     }
-  gj_invert<RG, CG, SMAX>(a, s, rowb, colb);
+  gj_invert<RG, CG, SMAX>(a, s, rowb, colb, ipb);
 #pragma unroll
   for (int r = 0; r < RPT; ++r)
 #pragma unroll
