@@ -28,8 +28,6 @@
 
 namespace {
 
-constexpr int SNT = 256;   // 4 waves, 2 x 2 over the output tile
-constexpr int STAGES = 4;
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 typedef __attribute__((address_space(1))) uint16_t gu16;
 typedef __attribute__((address_space(1))) float gfloat;
@@ -98,16 +96,22 @@ __device__ __forceinline__ StoreRef store_ref(CDesc* descs, int n_desc, int& di,
   return r;
 }
 
-// output staging: per wave 16 rows of 128 bytes (64 bf16 or 32 fp32), padded to 144 bytes
+// output staging: per wave 8 rows of 128 bytes (64 bf16 or 32 fp32), padded to 144 bytes
 constexpr int kStageRow = 144;
 
-template <int WM_T, int WN_T>
+// Tile configuration: WGM x WGN waves over the output tile, each wave WM_T x WN_T blocks of
+// 16 x 16; STAGES LDS stages.
+template <int WM_T_, int WN_T_, int WGM_, int WGN_, int STAGES_>
 struct StreamCfg {
-  static constexpr int BM = 32 * WM_T, BN = 32 * WN_T;
+  static constexpr int WM_T = WM_T_, WN_T = WN_T_, WGM = WGM_, WGN = WGN_, STAGES = STAGES_;
+  static constexpr int NW = WGM * WGN, NT = 64 * NW;
+  static constexpr int BM = 16 * WM_T * WGM, BN = 16 * WN_T * WGN;
   static constexpr int DA = BM / 8, DB = BN / 8;   // 1 KB load instructions per stage
-  static_assert((DA + DB) % 4 == 0, "load instructions must split evenly over the 4 waves");
-  static constexpr int D = (DA + DB) / 4;          // per wave per stage
+  static_assert((DA + DB) % NW == 0, "load instructions must split evenly over the waves");
+  static constexpr int D = (DA + DB) / NW;         // per wave per stage
   static constexpr int STAGE_BYTES = (BM + BN) * 128;
+  static_assert(D * (STAGES - 1) <= 63, "loads in flight exceed the vmcnt range");
+  static_assert(STAGES * STAGE_BYTES + NW * 8 * kStageRow <= 160 * 1024, "LDS budget");
 };
 
 // What the loads of one tile need: per load instruction q of the wave, the lane's source
@@ -122,10 +126,9 @@ struct LoadRef {
   bool live;
 };
 
-template <int WM_T, int WN_T>
-__device__ __forceinline__ LoadRef<StreamCfg<WM_T, WN_T>::D> load_ref(CDesc* descs, int n_desc, int& di, int64_t t,
-                                                                     int64_t total, int wave, int lane) {
-  using C = StreamCfg<WM_T, WN_T>;
+template <class C>
+__device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int& di, int64_t t, int64_t total,
+                                                  int wave, int lane) {
   LoadRef<C::D> r;
   if (t >= total) {
 #pragma unroll
@@ -143,7 +146,7 @@ __device__ __forceinline__ LoadRef<StreamCfg<WM_T, WN_T>::D> load_ref(CDesc* des
   const int64_t rows_a = d.M - pos.m0, rows_b = d.N - pos.n0;
 #pragma unroll
   for (int q = 0; q < C::D; ++q) {
-    const int i = wave + 4 * q;                     // load instruction index in the stage
+    const int i = wave + C::NW * q;                 // load instruction index in the stage
     const bool isA = i < C::DA;                     // uniform
     const int blk = isA ? i : i - C::DA;
     const int row = blk * 8 + (lane >> 3);
@@ -167,11 +170,11 @@ __device__ __forceinline__ void dma16(const void* src, uint32_t lds) {
 }
 
 // issue the loads of k-step kk of tile r into the stage at LDS address st
-template <int D>
-__device__ __forceinline__ void issue_stage(const LoadRef<D>& r, int kk, uint32_t st, int wave) {
+template <class C>
+__device__ __forceinline__ void issue_stage(const LoadRef<C::D>& r, int kk, uint32_t st, int wave) {
   const uint32_t kb = (uint32_t)kk * r.kstep;
 #pragma unroll
-  for (int q = 0; q < D; ++q) dma16(r.p[q] + kb, st + (wave + 4 * q) * 1024);
+  for (int q = 0; q < C::D; ++q) dma16(r.p[q] + kb, st + (wave + C::NW * q) * 1024);
 }
 
 // s_waitcnt vmcnt(n) then the workgroup barrier; a compiler memory fence, so no LDS access
@@ -182,10 +185,10 @@ __device__ __forceinline__ void wait_vm_barrier() {
   asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" : : "n"(N) : "memory");
 }
 
-template <int WM_T, int WN_T>
-__device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[WM_T][WN_T], int wm, int wn,
+template <class C>
+__device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[C::WM_T][C::WN_T], int wm, int wn,
                                               int lane) {
-  using C = StreamCfg<WM_T, WN_T>;
+  constexpr int WM_T = C::WM_T, WN_T = C::WN_T;
   const uint8_t* As = st;
   const uint8_t* Bs = st + C::BM * 128;
   u32x4 bf[WN_T][2];
@@ -222,43 +225,48 @@ __device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[
 }
 
 // Output of one tile through the wave's staging rows, for outputs of ESZ bytes: per 16-row
-// block and per 128-byte column group (64 bf16 or 32 fp32 outputs), each lane writes its
-// four consecutive n at row l % 16, then reads back 16 bytes of row h * 8 + l / 8 (chunk
-// l % 8), so one store covers 8 full 128-byte rows of C.  Rows that are not 16-byte aligned,
-// and the ragged n edge, are written element by element from the staged rows (a rolled loop).
-template <int ESZ, int WM_T, int WN_T>
-__device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[WM_T][WN_T], int wm, int wn, int lane,
-                                           uint8_t* so) {
-  static_assert(WN_T == 4, "a wave's 64 columns are 1 (bf16) or 2 (fp32) staged 128-byte groups");
+// block, per 128-byte column group (64 bf16 or 32 fp32 outputs) and per half of the block,
+// the lanes of that half write their four consecutive n at staging row l % 8, then every
+// lane reads back 16 bytes of row l / 8 (chunk l % 8), so one store covers 8 full 128-byte
+// rows of C.  Rows that are not 16-byte aligned, and the ragged n edge, are written element
+// by element from the staged rows (a rolled loop).
+template <int ESZ, class C>
+__device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C::WM_T][C::WN_T], int wm, int wn,
+                                           int lane, uint8_t* so) {
+  constexpr int WM_T = C::WM_T, WN_T = C::WN_T;
+  static_assert((WN_T * ESZ) % 8 == 0, "a wave's columns must fill whole 128-byte groups");
   constexpr int PER16 = 16 / ESZ;                  // outputs per 16-byte chunk
-  constexpr int GROUPS = ESZ / 2;                  // 128-byte column groups of the wave
-  constexpr int JPG = WN_T / GROUPS;               // 16-column blocks per group
+  constexpr int GROUPS = WN_T * ESZ / 8;           // 128-byte column groups of the wave
+  constexpr int JPG = 8 / ESZ;                     // 16-column blocks per group
   const float alpha = sr.alpha;
   const int64_t sCm = sr.sCm, sCn = sr.sCn, rows = sr.rows, cols = sr.cols;
+  const int half = (lane >> 3) & 1;                // which 8 of the block's 16 rows this lane holds
 #pragma unroll
   for (int i = 0; i < WM_T; ++i) {
     const int64_t mb = wm * WM_T * 16 + i * 16;
     if (mb < rows) {                               // uniform
 #pragma unroll
       for (int g = 0; g < GROUPS; ++g) {
-#pragma unroll
-        for (int jj = 0; jj < JPG; ++jj) {
-          const floatx4 v = acc[i][g * JPG + jj] * alpha;
-          uint8_t* dst = so + (lane & 15) * kStageRow + (jj * 16 + (lane >> 4) * 4) * ESZ;
-          if (ESZ == 2) {
-            u32x2 w;
-            w.x = (uint32_t)fh_f32_to_bf16_bits(v[0]) | ((uint32_t)fh_f32_to_bf16_bits(v[1]) << 16);
-            w.y = (uint32_t)fh_f32_to_bf16_bits(v[2]) | ((uint32_t)fh_f32_to_bf16_bits(v[3]) << 16);
-            *(u32x2*)dst = w;
-          } else {
-            *(floatx4*)dst = v;
-          }
-        }
         const int64_t nb = wn * WN_T * 16 + g * (JPG * 16);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int row = h * 8 + (lane >> 3), ch = lane & 7;
-          const int64_t m = mb + row, n = nb + ch * PER16;
+        for (int hh = 0; hh < 2; ++hh) {
+          if (half == hh) {
+#pragma unroll
+            for (int jj = 0; jj < JPG; ++jj) {
+              const floatx4 v = acc[i][g * JPG + jj] * alpha;
+              uint8_t* dst = so + (lane & 7) * kStageRow + (jj * 16 + (lane >> 4) * 4) * ESZ;
+              if (ESZ == 2) {
+                u32x2 w;
+                w.x = (uint32_t)fh_f32_to_bf16_bits(v[0]) | ((uint32_t)fh_f32_to_bf16_bits(v[1]) << 16);
+                w.y = (uint32_t)fh_f32_to_bf16_bits(v[2]) | ((uint32_t)fh_f32_to_bf16_bits(v[3]) << 16);
+                *(u32x2*)dst = w;
+              } else {
+                *(floatx4*)dst = v;
+              }
+            }
+          }
+          const int row = lane >> 3, ch = lane & 7;
+          const int64_t m = mb + hh * 8 + row, n = nb + ch * PER16;
           const uint8_t* srcrow = so + row * kStageRow + ch * 16;
           if (sr.vec && n + PER16 <= cols) {
             const u32x4 v = *(const u32x4*)srcrow;
@@ -279,28 +287,29 @@ __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[WM
   }
 }
 
-// One barrier per k-step: at step k every wave has waited for its loads of stage k % 4 and
-// finished step k - 1, so the stage read at step k - 1 is refilled with step k + 3 (the load
-// cursor runs STAGES - 1 steps ahead, through zero blocks past the last tile).  The output
+// One barrier per k-step: at step k every wave has waited for its loads of stage k % STAGES
+// and finished step k - 1, so the stage read at step k - 1 is refilled with step
+// k + STAGES - 1 (the load cursor runs STAGES - 1 steps ahead, through zero blocks past the
+// last tile).  The output
 // stores count in vmcnt too; the stage wait stays exact because a wave's loads retire in
 // issue order, so vmcnt <= n leaves at most its n youngest loads outstanding.
-template <int WM_T, int WN_T>
-__global__ __launch_bounds__(SNT) void gemm_stream_kernel(const FhGemmDesc* __restrict__ descs_arg, int n_desc,
-                                                          int64_t total_tiles) {
-  using C = StreamCfg<WM_T, WN_T>;
+template <class C>
+__global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __restrict__ descs_arg, int n_desc,
+                                                            int64_t total_tiles) {
+  constexpr int WM_T = C::WM_T, WN_T = C::WN_T, STAGES = C::STAGES;
   CDesc* descs = (CDesc*)descs_arg;
   __shared__ __attribute__((aligned(1024))) uint8_t ring[STAGES * C::STAGE_BYTES];
-  __shared__ __attribute__((aligned(16))) uint8_t SO[4][16 * kStageRow];
+  __shared__ __attribute__((aligned(16))) uint8_t SO[C::NW][8 * kStageRow];
   const uint32_t ring_lds = (uint32_t)(uintptr_t)(lds_ptr_t)ring;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % C::WGM, wn = wave / C::WGM;
   const int64_t stride = gridDim.x;
 
   // load cursor
   int64_t lt = blockIdx.x;
   int dl = 0;
-  LoadRef<C::D> lr = load_ref<WM_T, WN_T>(descs, n_desc, dl, lt, total_tiles, wave, lane);
+  LoadRef<C::D> lr = load_ref<C>(descs, n_desc, dl, lt, total_tiles, wave, lane);
   int lk = 0;
   // compute cursor
   int64_t ct = blockIdx.x;
@@ -315,23 +324,22 @@ __global__ __launch_bounds__(SNT) void gemm_stream_kernel(const FhGemmDesc* __re
     for (int j = 0; j < WN_T; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   auto issue = [&](int stage) {
-    issue_stage(lr, lk, ring_lds + stage * C::STAGE_BYTES, wave);
+    issue_stage<C>(lr, lk, ring_lds + stage * C::STAGE_BYTES, wave);
     if (++lk >= lr.nk) {
       lk = 0;
       lt += stride;
-      lr = load_ref<WM_T, WN_T>(descs, n_desc, dl, lt, total_tiles, wave, lane);
+      lr = load_ref<C>(descs, n_desc, dl, lt, total_tiles, wave, lane);
     }
   };
-  issue(0);
-  issue(1);
-  issue(2);
+#pragma unroll
+  for (int st = 0; st < STAGES - 1; ++st) issue(st);
   for (int k = 0; sr.live; ++k) {
     wait_vm_barrier<C::D * (STAGES - 2)>();
     issue((k + STAGES - 1) % STAGES);
-    compute_stage<WM_T, WN_T>(ring + (k % STAGES) * C::STAGE_BYTES, acc, wm, wn, lane);
+    compute_stage<C>(ring + (k % STAGES) * C::STAGE_BYTES, acc, wm, wn, lane);
     if (++ck >= sr.nk) {
-      if (sr.bf16) store_tile<2>(sr, acc, wm, wn, lane, SO[wave]);
-      else store_tile<4>(sr, acc, wm, wn, lane, SO[wave]);
+      if (sr.bf16) store_tile<2, C>(sr, acc, wm, wn, lane, SO[wave]);
+      else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);
 #pragma unroll
       for (int i = 0; i < WM_T; ++i)
 #pragma unroll
@@ -347,14 +355,23 @@ __global__ __launch_bounds__(SNT) void gemm_stream_kernel(const FhGemmDesc* __re
 }  // namespace
 
 // Streaming bf16 GEMM over a descriptor table prepared by fh_gemm_prepare(descs, n, bm, bn).
-// bm in {64, 96, 128, 160}, bn = 128.  grid: persistent workgroups (0 = one per CU).
+// (bm, bn): (64 | 96 | 128 | 160, 128) with 4 waves and 4 stages, or (128, 256) with 8 waves
+// and 3 stages.  grid: persistent workgroups (0 = one per CU).
+namespace {
+template <class C>
+int launch_stream(const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL((gemm_stream_kernel<C>), dim3(grid), dim3(C::NT), 0, stream, descs_dev, n_desc, total_tiles);
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+}  // namespace
+
 extern "C" int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                                      int grid, hipStream_t stream) {
   if (total_tiles <= 0) return FH_OK;
   FH_REQUIRE(descs_dev != nullptr && n_desc > 0, "gemm_stream: descriptor table required");
   FH_REQUIRE(total_tiles < ((int64_t)1 << 31), "gemm_stream: %lld tiles exceed the 32-bit tile index",
              (long long)total_tiles);
-  FH_REQUIRE(bn == 128, "gemm_stream: bn must be 128 (got %d)", bn);
   if (grid <= 0) {
     int dev = 0, cus = 0;
     FH_CHECK_HIP(hipGetDevice(&dev));
@@ -362,23 +379,17 @@ extern "C" int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev
     grid = cus;
   }
   if ((int64_t)grid > total_tiles) grid = (int)total_tiles;
-  switch (bm) {
-    case 64:
-      hipLaunchKernelGGL((gemm_stream_kernel<2, 4>), dim3(grid), dim3(SNT), 0, stream, descs_dev, n_desc, total_tiles);
-      break;
-    case 96:
-      hipLaunchKernelGGL((gemm_stream_kernel<3, 4>), dim3(grid), dim3(SNT), 0, stream, descs_dev, n_desc, total_tiles);
-      break;
-    case 128:
-      hipLaunchKernelGGL((gemm_stream_kernel<4, 4>), dim3(grid), dim3(SNT), 0, stream, descs_dev, n_desc, total_tiles);
-      break;
-    case 160:
-      hipLaunchKernelGGL((gemm_stream_kernel<5, 4>), dim3(grid), dim3(SNT), 0, stream, descs_dev, n_desc, total_tiles);
-      break;
-    default:
-      fh_set_error("gemm_stream: unsupported tile height %d (64, 96, 128 or 160)", bm);
-      return FH_ERR_ARG;
+  if (bn == 128) {
+    switch (bm) {
+      case 64: return launch_stream<StreamCfg<2, 4, 2, 2, 4>>(descs_dev, n_desc, total_tiles, grid, stream);
+      case 96: return launch_stream<StreamCfg<3, 4, 2, 2, 4>>(descs_dev, n_desc, total_tiles, grid, stream);
+      case 128: return launch_stream<StreamCfg<4, 4, 2, 2, 4>>(descs_dev, n_desc, total_tiles, grid, stream);
+      case 160: return launch_stream<StreamCfg<5, 4, 2, 2, 4>>(descs_dev, n_desc, total_tiles, grid, stream);
+      default: break;
+    }
+  } else if (bn == 256 && bm == 128) {
+    return launch_stream<StreamCfg<4, 4, 2, 4, 3>>(descs_dev, n_desc, total_tiles, grid, stream);
   }
-  FH_CHECK_LAUNCH();
-  return FH_OK;
+  fh_set_error("gemm_stream: unsupported tile %d x %d", bm, bn);
+  return FH_ERR_ARG;
 }

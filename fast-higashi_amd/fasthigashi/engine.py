@@ -314,7 +314,8 @@ class SweepEngine:
         self.meta_ksplit = meta_ksplit
         self.SVDt = z(self.Npad, self.R)
         self.GMpart = z(meta_ksplit * self.R * self.R, dt=torch.float64)
-        self.M16 = z(self.Npad, self.R, dt=torch.bfloat16) if dtype == "bf16" else None
+        # bf16 path: M^T (R x cells, cells contiguous) for the streamed T_tilde = T_hat M
+        self.M16t = z(self.R, self.Npad, dt=torch.bfloat16) if dtype == "bf16" else None
         self.Ttil = z(ttoff)
         # polar batches: per-bin (one group per chromosome) and the meta-embedding
         self.pb = ops.PolarBatch([c.s for c in self.chroms], [c.n for c in self.chroms], ns_iter, ns_tol, dev)
@@ -525,13 +526,22 @@ class SweepEngine:
         d13 = make_desc(self.SVDt.data_ptr(), Zm.data_ptr(), self.M.data_ptr(), Np, R, R, R, 1,
                         Zm.stride(1), 1, R, 1)
         self.g_m = ops.Gemm64Group([d13], F32, F64, F32, dev)
-        # (14) T_tilde_c = T_hat_c M
-        Mp = self.M16.data_ptr() if self.dtype == "bf16" else self.M.data_ptr()
+        # (14) T_tilde_c = T_hat_c M.  bf16: streaming kernel over T_hat (k = cells contiguous)
+        # against M^T, one 128 x 256 tile column for R <= 256, so T_hat is read once
         d14 = []
-        for cl in self.chroms:
-            d14.append(make_desc(Thp + xe * cl.toff * Np, Mp, f4(self.Ttil, cl.ttoff), cl.n * cl.sp, R, Np,
-                                 Np, 1, R, 1, R, 1))
-        self.g_ttil = MultiGemm(d14, self.xdt, lambda d: 6 if d.N > 160 else shape_for_n(d.N), dev)
+        if self.dtype == "bf16":
+            self.g_mt = ops.TransposeGroup([(self.M.data_ptr(), F32, self.M16t.data_ptr(), BF16, Np, R, R, Np,
+                                             1, 0, 0)], dev)
+            for cl in self.chroms:
+                d14.append(make_desc(Thp + xe * cl.toff * Np, self.M16t.data_ptr(), f4(self.Ttil, cl.ttoff),
+                                     cl.n * cl.sp, R, Np, Np, 1, 1, Np, R, 1))
+            self.g_ttil = StreamMulti([ops.StreamGemmGroup(d14, bm=128, bn=128 if R <= 128 else 256,
+                                                           device=dev)])
+        else:
+            for cl in self.chroms:
+                d14.append(make_desc(Thp + xe * cl.toff * Np, self.M.data_ptr(), f4(self.Ttil, cl.ttoff),
+                                     cl.n * cl.sp, R, Np, Np, 1, R, 1, R, 1))
+            self.g_ttil = MultiGemm(d14, self.xdt, lambda d: 6 if d.N > 160 else shape_for_n(d.N), dev)
         self._build_cpals()
 
     def _build_cpals(self):
@@ -627,8 +637,7 @@ class SweepEngine:
         self.pbm.solve(stream)
         self.g_m.run(stream)
         if self.dtype == "bf16":
-            L.call("fh_copy2d", self.M.data_ptr(), F32, self.R, 1, self.M16.data_ptr(), BF16, self.R,
-                   self.Npad, self.R, 1.0, st)
+            self.g_mt.run(stream)
         # pass 3
         mk("ttilde")
         self.g_ttil.run(stream)

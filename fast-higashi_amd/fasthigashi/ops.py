@@ -152,9 +152,10 @@ def stream_bm(M: int) -> int:
 
 class StreamGemmGroup:
     """Grouped launch of the streaming bf16 GEMM (both operands k-contiguous, persistent
-    workgroups, 4-deep global->LDS ring); tiles bm x 128."""
+    workgroups, global->LDS ring); tiles bm x bn: bm in STREAM_BM with bn = 128, or 128 x 256."""
 
-    def __init__(self, descs: Sequence[L.GemmDesc], bm: Optional[int] = None, grid: int = 0, device="cuda"):
+    def __init__(self, descs: Sequence[L.GemmDesc], bm: Optional[int] = None, grid: int = 0, device="cuda",
+                 bn: int = 128):
         assert len(descs) > 0
         for d in descs:
             assert d.sAk == 1 and d.sBk == 1, "streaming GEMM needs k-contiguous A and B"
@@ -166,15 +167,17 @@ class StreamGemmGroup:
             assert d.beta == 0.0 and d.diag == 0.0 and (d.flags & ~L.GEMM_OUT_BF16) == 0, \
                 "streaming GEMM: flags 0 or OUT_BF16, no beta/diag"
         self.bm = bm if bm is not None else stream_bm(max(d.M for d in descs))
+        self.bn = bn
+        assert (bn == 128 and self.bm in STREAM_BM) or (bn, self.bm) == (256, 128), "tile %dx%d" % (self.bm, bn)
         self.grid = grid
         arr = (L.GemmDesc * len(descs))(*descs)
-        self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), self.bm, 128)
+        self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), self.bm, bn)
         self.n = len(descs)
         self.descs = list(arr)
         self.table = _upload(arr, device)
 
     def run(self, stream=None):
-        L.call("fh_gemm_stream_launch", self.bm, 128, self.table.data_ptr(), self.n, self.total, self.grid,
+        L.call("fh_gemm_stream_launch", self.bm, self.bn, self.table.data_ptr(), self.n, self.total, self.grid,
                stream_handle(stream))
 
 

@@ -310,6 +310,22 @@ def test_gemm_stream_matches_reference(cuda, grid):
             assert bool((C[:, M:].float() == 7.0).all()), (bm, case)
 
 
+def test_gemm_stream_wide_tile(cuda):
+    """128 x 256 tiles (8 waves, 3 stages): the T_tilde shape (long k, N = rank 256) and ragged
+    cases, fp32 and bf16 output."""
+    from fasthigashi import ops
+    gen = torch.Generator().manual_seed(37)
+    cases = [(300, 256, 1000, 1, False), (130, 200, 64, 3, True), (77, 256, 130, 2, False), (256, 256, 640, 1, True)]
+    built = [_stream_case(cuda, gen, M, N, K, b, o) for (M, N, K, b, o) in cases]
+    ops.StreamGemmGroup([x[0] for x in built], bm=128, bn=256, device=cuda).run()
+    torch.cuda.synchronize()
+    for (d, C, ref, M, _), case in zip(built, cases):
+        got = C[:, :M].double().cpu()
+        err = (got - ref).abs().max().item() / ref.abs().max().item()
+        assert err < (1e-2 if case[4] else 1e-5), (case, err)
+        assert bool((C[:, M:].float() == 7.0).all()), case
+
+
 def test_gemm_stream_matches_tiled_gemm(cuda):
     """same descriptors through the streaming kernel and the general tiled bf16 GEMM."""
     from fasthigashi import ops, _lib as L
