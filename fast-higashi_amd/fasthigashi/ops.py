@@ -328,7 +328,9 @@ class PolarBatch:
         return self.nuc[m: m + self.counts[g]]
 
     newton_iter = 12          # scaled Newton steps at most (about log2(log(cond G)) + 3 are used)
-    newton_tol = 1e-20        # stop when ||H - I||_F^2 < newton_tol
+    # stop when ||H - I||_F^2 < newton_tol: ||H - I|| < 1e-5 leaves K = G^{-1/2} (I + O(1e-5)),
+    # and the closing Newton-Schulz correction on the unshifted G squares that to O(1e-10)
+    newton_tol = 1e-10
 
     def solve_fused(self, stream=None, method="newton"):
         """One workgroup per matrix; G^{-1/2} ends in buffer 0.  method 'newton': scaled Newton

@@ -236,7 +236,9 @@ def test_polar_fused_matches_multilaunch(cuda):
 @pytest.mark.parametrize("log_kappa", [0.5, 6.0, 11.0])
 def test_polar_newton_matches_eigh(cuda, log_kappa):
     """Scaled Newton G^{-1/2} (every size class, plus the Newton-Schulz fallback above 160)
-    against the FP64 eigendecomposition, and its nuclear norm tr(G^{1/2})."""
+    against the FP64 eigendecomposition, and its nuclear norm tr(G^{1/2}).  The solve stops at
+    ||H - I||_F < 1e-5 and closes with one Newton-Schulz correction (error squared), so the
+    bounds allow 1e-9 besides the conditioning term."""
     from fasthigashi import ops
     rng = np.random.default_rng(int(log_kappa * 10))
     sizes, counts = [1, 9, 64, 65, 128, 129, 150, 160, 161], [2, 3, 2, 2, 1, 1, 2, 1, 1]
@@ -255,9 +257,9 @@ def test_polar_newton_matches_eigh(cuda, log_kappa):
             w, V = np.linalg.eigh(G[b])
             ref = (V / np.sqrt(w)) @ V.T
             cond = w.max() / w.min()
-            assert np.abs(Z[b] - ref).max() <= 1e-13 * cond * np.abs(ref).max(), (s, b, cond)
+            assert np.abs(Z[b] - ref).max() <= (1e-13 * cond + 1e-9) * np.abs(ref).max(), (s, b, cond)
             # the polar factor's orthogonality: Z G Z = I
-            assert np.abs(Z[b] @ G[b] @ Z[b] - np.eye(s)).max() <= 1e-14 * cond + 1e-12, (s, b)
+            assert np.abs(Z[b] @ G[b] @ Z[b] - np.eye(s)).max() <= 1e-14 * cond + 1e-9, (s, b)
         nuc = pb.nuc_of(g).cpu().numpy()
         ref_nuc = [np.sqrt(np.clip(np.linalg.eigvalsh(x), 0, None)).sum() for x in G]
         assert np.allclose(nuc, ref_nuc, rtol=1e-9)
