@@ -421,12 +421,20 @@ class SweepEngine:
             d1.append(make_desc(self.M.data_ptr(), f4(self.D_all, cl.doff), Ctp + xe * cl.doff * Np,
                                 Np, cl.s, R, R, 1, Dld, 1, 1, Np, flags=out16))
         self.g_ct = MultiGemm(d1, F32, lambda d: shape_for_n(d.N), dev)
-        # (2) Y_c = X_c C_c
+        # (2) Y_c = X_c C_c.  bf16: streaming kernel over X (positions x cells, cells contiguous)
+        # against C_c^T (s x cells), 128 x 128 tiles for s <= 128 and 128 x 256 above, so X is
+        # read once
         d2 = []
         for cl in self.chroms:
             d2.append(make_desc(Xp + xe * cl.xrow * Np, Ctp + xe * cl.doff * Np, f4(self.Y, cl.yoff),
                                 cl.W, cl.s, Np, Np, 1, 1, Np, cl.sp, 1))
-        self.g_y = MultiGemm(d2, self.xdt, lambda d: shape_for_n(d.N), dev)
+        if self.dtype == "bf16":
+            narrow = [d for d in d2 if d.N <= 128]
+            wide = [d for d in d2 if d.N > 128]
+            self.g_y = StreamMulti([ops.StreamGemmGroup(v, bm=128, bn=bn, device=dev)
+                                    for v, bn in ((narrow, 128), (wide, 256)) if v])
+        else:
+            self.g_y = MultiGemm(d2, self.xdt, lambda d: shape_for_n(d.N), dev)
         # (3) Y rows *= A[bin]
         rs = []
         for c, cl in enumerate(self.chroms):
