@@ -295,12 +295,17 @@ def synth_device(wl, device="cuda", cell_batch: int = 1024) -> List[DeviceCOO]:
         cells = torch.cat([ce, ce[off]])
         vals = torch.cat([torch.where(off, cn, cn + cn), cn[off]])
         keep = (cols - rows).abs() <= wl.off_diag
+        n_first = int(keep[: ce.shape[0]].sum().item())
         rows, cols, cells, vals = rows[keep], cols[keep], cells[keep], vals[keep]
         valid = torch.bincount(rows, minlength=n) > 0
         mapping = torch.cumsum(valid.to(torch.int64), 0) - 1
         mapping[~valid] = -1
         n_valid = int(valid.sum().item())
-        sums = torch.zeros(wl.n_cells, dtype=torch.float64, device=dev).index_add_(0, cells, vals.double())
+        # per-cell sums: the keys are sorted by cell, so cells is two ascending runs (the
+        # entries, then the mirrored off-diagonal ones); FP64 prefix sums differenced at each
+        # run's cell boundaries, no atomics
+        sums = _sorted_segment_sum(cells[:n_first], vals[:n_first], wl.n_cells) + \
+            _sorted_segment_sum(cells[n_first:], vals[n_first:], wl.n_cells)
         factor = torch.tensor(np.float32(n), device=dev) / (sums.float() + 1e-15)
         vals = vals * factor[cells]
         r, cc = mapping[rows], mapping[cols]
@@ -315,6 +320,14 @@ def synth_device(wl, device="cuda", cell_batch: int = 1024) -> List[DeviceCOO]:
         del idx, vals
         res.append(DeviceCOO(oi, ov, ptr, ptr.cpu().numpy(), n_valid, wl.n_cells))
     return res
+
+
+def _sorted_segment_sum(keys: torch.Tensor, vals: torch.Tensor, n: int) -> torch.Tensor:
+    """FP64 sums of vals per key 0..n-1 for ascending keys."""
+    cs = torch.zeros(keys.shape[0] + 1, dtype=torch.float64, device=keys.device)
+    torch.cumsum(vals.double(), 0, out=cs[1:])
+    bounds = torch.searchsorted(keys, torch.arange(n + 1, device=keys.device, dtype=keys.dtype))
+    return cs[bounds[1:]] - cs[bounds[:-1]]
 
 
 def coo_to_device(idx: np.ndarray, val: np.ndarray, shape, device="cuda", presorted=False) -> DeviceCOO:
