@@ -586,9 +586,10 @@ __global__ __launch_bounds__(256) void newton_init_kernel(const FhNsGroup* __res
 
 // register grid of the in-place inverse per size class: RG x CG threads, SMAX = matrix bound
 template <int C> struct GjCfg;
-template <> struct GjCfg<2> { static constexpr int RG = 16, CG = 16, SMAX = 64; };
-template <> struct GjCfg<4> { static constexpr int RG = 16, CG = 32, SMAX = 128; };
-template <> struct GjCfg<5> { static constexpr int RG = 16, CG = 32, SMAX = 160; };
+// WPE: waves per SIMD the register budget is sized for (4: two 8-wave workgroups per CU)
+template <> struct GjCfg<2> { static constexpr int RG = 16, CG = 16, SMAX = 64, WPE = 4; };
+template <> struct GjCfg<4> { static constexpr int RG = 16, CG = 32, SMAX = 128, WPE = 4; };
+template <> struct GjCfg<5> { static constexpr int RG = 16, CG = 32, SMAX = 160, WPE = 2; };
 
 // In-place Gauss-Jordan inverse of an SPD s x s matrix (s <= SMAX) held by the workgroup's
 // RG x CG threads: element (i, j) in a[i / RG][j / CG] of thread (i % RG, j % CG), padding
@@ -660,7 +661,8 @@ __device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int
 // backward stable where an explicit inverse of a pivot block is not.  Rows/columns >= s are
 // identity padding.
 template <int WT>
-__global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void newton_gj_kernel(
+__global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) __attribute__((amdgpu_waves_per_eu(GjCfg<WT>::WPE)))
+void newton_gj_kernel(
     const FhNsGroup* __restrict__ groups, int n_groups, const int64_t* __restrict__ mat_begin,
     int* __restrict__ state, int step, double tol) {
   constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX, NTH = RG * CG;
