@@ -287,15 +287,95 @@ __global__ __launch_bounds__(NT) void gemm_mfma_kernel(const FhGemmDesc* __restr
 }
 
 // ---------------------------------------------------------------------------
-// fp64: 64 x 64 x 16 tiles, 4 waves each 32 x 32 (2 x 2 MFMA f64 16x16x4)
-// A, B may be fp32 (converted exactly) or fp64; C fp64 or fp32.
+// fp64: 64 x 64 output tiles, k-tiles of 32, 4 waves each 32 x 32 (2 x 2 MFMA f64 16x16x4).
+// A, B may be fp32 (converted exactly) or fp64; C fp64 or fp32.  Each k-tile of an operand
+// (64 rows x 32 k = 2048 values) is loaded as 8 consecutive values per thread along the
+// operand's unit-stride dimension (vector loads where the run is whole and aligned, scalar
+// otherwise), held in registers while the previous k-tile is multiplied, then stored to LDS.
 // ---------------------------------------------------------------------------
+constexpr int F64_BK = 32, F64_LR = F64_BK + 4;   // LDS row: 32 k + 4 padding doubles
+
+// the 8 values of one thread's run of a 64 x 32 operand tile (rows x k), into double regs
+template <typename T>
+struct RunLoader {
+  const T* base;      // element (row 0, k 0) of the tile's operand (batch applied)
+  int64_t s_row, s_k; // strides
+  int64_t rows;       // valid rows from row 0
+  bool kfast;         // unit stride along k (else along rows)
+  bool vec;           // runs may use vector loads (stride and base alignment)
+
+  // run r of thread t: kfast -> row t / 4, k (t % 4) * 8 .. +8; else k t / 8, rows (t % 8) * 8 .. +8
+  __device__ __forceinline__ void load(T (&v)[8], int tid, int64_t k0, int64_t kend) const {
+    const int a = kfast ? tid >> 2 : (tid & 7) * 8;       // row (or first row)
+    const int b = kfast ? (tid & 3) * 8 : tid >> 3;       // first k (or k)
+    const int64_t k = k0 + b;
+    if (kfast) {
+      const bool whole = a < rows && k + 8 <= kend;
+      const T* p = base + a * s_row + k;
+      if (whole && vec) {
+        load8(v, p);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (a < rows && k + e < kend) ? p[e] : (T)0;
+      }
+    } else {
+      const bool whole = a + 8 <= rows && k < kend;
+      const T* p = base + (int64_t)a * s_row + k * s_k;
+      if (whole && vec) {
+        load8(v, p);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (a + e < rows && k < kend) ? p[e * s_row] : (T)0;
+      }
+    }
+  }
+  static __device__ __forceinline__ void load8(T (&v)[8], const T* p);
+  // LDS position of the run (as doubles): element (row, k) at row * F64_LR + k
+  __device__ __forceinline__ void store(const T (&v)[8], double* lds, int tid) const {
+    if (kfast) {
+      double* q = lds + (tid >> 2) * F64_LR + (tid & 3) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) q[e] = (double)v[e];
+    } else {
+      double* q = lds + ((tid & 7) * 8) * F64_LR + (tid >> 3);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) q[e * F64_LR] = (double)v[e];
+    }
+  }
+};
+template <>
+__device__ __forceinline__ void RunLoader<float>::load8(float (&v)[8], const float* p) {
+  const floatx4 x = *(const floatx4*)p, y = *(const floatx4*)(p + 4);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v[e] = x[e];
+    v[4 + e] = y[e];
+  }
+}
+template <>
+__device__ __forceinline__ void RunLoader<double>::load8(double (&v)[8], const double* p) {
+  typedef double doublex2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const doublex2 x = *(const doublex2*)(p + 2 * e);
+    v[2 * e] = x[0];
+    v[2 * e + 1] = x[1];
+  }
+}
+
+// vector runs need the run's start 16-byte aligned: base aligned and the stride between runs
+// (the non-unit stride) a multiple of 16 bytes
+template <typename T>
+__device__ __forceinline__ bool runs_aligned(const T* base, int64_t other_stride) {
+  return (((uintptr_t)base) & 15) == 0 && ((other_stride * (int64_t)sizeof(T)) & 15) == 0;
+}
+
 template <typename TA, typename TB, typename TC>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restrict__ descs,
                                                       int n_desc, FhGemmDesc single) {
-  constexpr int BM = 64, BN = 64, BKD = 16, LR = BKD + 1;
-  __shared__ double As[BM * LR];
-  __shared__ double Bs[BN * LR];
+  constexpr int BM = 64, BN = 64;
+  __shared__ double As[BM * F64_LR];
+  __shared__ double Bs[BN * F64_LR];
   const int64_t t = blockIdx.x;
   const FhGemmDesc* dp = &single;
   if (descs != nullptr) dp = &descs[find_desc(descs, n_desc, t)];
@@ -311,15 +391,27 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restri
   local /= ksplit;
   const int64_t tm = local / d.tiles_n, tn = local % d.tiles_n;
   const int64_t m0 = tm * BM, n0 = tn * BN;
-  const int64_t kchunk = ((d.K + ksplit - 1) / ksplit + BKD - 1) / BKD * BKD;
+  const int64_t kchunk = ((d.K + ksplit - 1) / ksplit + F64_BK - 1) / F64_BK * F64_BK;
   const int64_t kbeg = ks * kchunk;
   const int64_t kend = kbeg + kchunk < d.K ? kbeg + kchunk : d.K;
 
-  const TA* A = (const TA*)d.A + b * d.sAb;
-  const TB* B = (const TB*)d.B + b * d.sBb;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const bool a_k_fast = d.sAk == 1, b_k_fast = d.sBk == 1;
+  // A as rows m x k; B as rows n x k
+  RunLoader<TA> la;
+  la.base = (const TA*)d.A + b * d.sAb + m0 * d.sAm;
+  la.s_row = d.sAm;
+  la.s_k = d.sAk;
+  la.rows = d.M - m0;
+  la.kfast = d.sAk == 1;
+  la.vec = runs_aligned(la.base, la.kfast ? d.sAm : d.sAk);
+  RunLoader<TB> lb;
+  lb.base = (const TB*)d.B + b * d.sBb + n0 * d.sBn;
+  lb.s_row = d.sBn;
+  lb.s_k = d.sBk;
+  lb.rows = d.N - n0;
+  lb.kfast = d.sBk == 1;
+  lb.vec = runs_aligned(lb.base, lb.kfast ? d.sBn : d.sBk);
 
   doublex4 acc[2][2];
 #pragma unroll
@@ -327,30 +419,28 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restri
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = doublex4{0., 0., 0., 0.};
 
-  for (int64_t k0 = kbeg; k0 < kend; k0 += BKD) {
+  TA ra[8];
+  TB rb[8];
+  if (kbeg < kend) {
+    la.load(ra, tid, kbeg, kend);
+    lb.load(rb, tid, kbeg, kend);
+  }
+  for (int64_t k0 = kbeg; k0 < kend; k0 += F64_BK) {
+    __syncthreads();                       // previous k-tile consumed
+    la.store(ra, As, tid);
+    lb.store(rb, Bs, tid);
     __syncthreads();
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      int e = tid + p * NT;
-      int m, k;
-      if (a_k_fast) { m = e >> 4; k = e & 15; } else { m = e & 63; k = e >> 6; }
-      double v = 0.;
-      if (m0 + m < d.M && k0 + k < kend) v = (double)A[(m0 + m) * d.sAm + (k0 + k) * d.sAk];
-      As[m * LR + k] = v;
-      int n;
-      if (b_k_fast) { n = e >> 4; k = e & 15; } else { n = e & 63; k = e >> 6; }
-      double w = 0.;
-      if (n0 + n < d.N && k0 + k < kend) w = (double)B[(k0 + k) * d.sBk + (n0 + n) * d.sBn];
-      Bs[n * LR + k] = w;
+    if (k0 + F64_BK < kend) {              // next k-tile in flight during this one's MFMAs
+      la.load(ra, tid, k0 + F64_BK, kend);
+      lb.load(rb, tid, k0 + F64_BK, kend);
     }
-    __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < BKD; kk += 4) {
+    for (int kk = 0; kk < F64_BK; kk += 4) {
       double a[2], bb[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[(wm * 32 + i * 16 + (lane & 15)) * LR + kk + (lane >> 4)];
+      for (int i = 0; i < 2; ++i) a[i] = As[(wm * 32 + i * 16 + (lane & 15)) * F64_LR + kk + (lane >> 4)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bb[j] = Bs[(wn * 32 + j * 16 + (lane & 15)) * LR + kk + (lane >> 4)];
+      for (int j = 0; j < 2; ++j) bb[j] = Bs[(wn * 32 + j * 16 + (lane & 15)) * F64_LR + kk + (lane >> 4)];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll

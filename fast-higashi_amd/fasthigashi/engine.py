@@ -241,7 +241,7 @@ class SweepEngine:
 
     def __init__(self, tiles: Sequence[Sequence[TileSpec]], n_bins: Sequence[int], sizes: Sequence[int],
                  n_cells: int, rank: int, dtype: str = "fp32", device="cuda", names=None,
-                 ns_iter: int = 48, ns_tol: float = 1e-24, meta_ksplit: int = 32, n_iter_max_hist: int = 512):
+                 ns_iter: int = 48, ns_tol: float = 1e-20, meta_ksplit: int = 32, n_iter_max_hist: int = 512):
         assert dtype in ("fp32", "bf16")
         self.dtype = dtype
         self.xdt = F32 if dtype == "fp32" else BF16
@@ -563,6 +563,25 @@ class SweepEngine:
     def _core_norms(self, which: int, stream=None):
         self.cpals.core_norms(which, stream)
 
+    @staticmethod
+    def group_tag(g) -> str:
+        """phase-name suffix of one grouped launch: s<bm>x<bn> (streaming) or t<shape> (tiled)."""
+        return ("s%dx%d" % (g.bm, g.bn)) if isinstance(g, ops.StreamGemmGroup) else ("t%d" % g.shape)
+
+    def launch_groups(self) -> dict:
+        """phase name -> grouped launch, for the launches timed one by one (Y, T_hat, T_tilde)."""
+        out = {}
+        for prefix, multi in (("y", self.g_y), ("that", self.g_that), ("ttilde", self.g_ttil)):
+            for g in multi.groups:
+                out["%s:%s" % (prefix, self.group_tag(g))] = g
+        return out
+
+    def _run_marked(self, prefix, multi, stream):
+        # one marker per launch, so each launch is timed on its own
+        for g in multi.groups:
+            self._mark("%s:%s" % (prefix, self.group_tag(g)), stream)
+            g.run(stream)
+
     def _mark(self, name, stream):
         if self.timing is not None:
             ev = torch.cuda.Event(enable_timing=True)
@@ -611,8 +630,7 @@ class SweepEngine:
         # pass 1
         mk("ct")
         self.g_ct.run(stream)
-        mk("mttkrp_y")
-        self.g_y.run(stream)
+        self._run_marked("y", self.g_y, stream)
         mk("scale_temp")
         L.call("fh_scale_rows", self.rs_tab.data_ptr(), self.rs_n, self.rs_max, st)
         self.g_temp.run(stream)
@@ -625,10 +643,8 @@ class SweepEngine:
         L.call("fh_segment_sum", self.pb.nuc.data_ptr(), self.seg.data_ptr(), self.K, self.xU.data_ptr(), 0, st)
         if self.dtype == "bf16":
             self.g_pt.run(stream)
-        # pass 2 (one marker per tile-shape launch, so each launch is timed on its own)
-        for g in self.g_that.groups:
-            mk("that:%s" % (("b%d" % g.bm) if isinstance(g, ops.StreamGemmGroup) else g.shape))
-            g.run(stream)
+        # pass 2
+        self._run_marked("that", self.g_that, stream)
         mk("krp")
         for c, cl in enumerate(self.chroms):
             L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,
@@ -647,8 +663,7 @@ class SweepEngine:
         if self.dtype == "bf16":
             self.g_mt.run(stream)
         # pass 3
-        mk("ttilde")
-        self.g_ttil.run(stream)
+        self._run_marked("ttilde", self.g_ttil, stream)
 
     def transform(self, stream=None) -> np.ndarray:
         """Final meta-embedding (parafac2_intergrative.py:742-834): SVD_term from the stored
