@@ -591,6 +591,16 @@ template <> struct GjCfg<2> { static constexpr int RG = 16, CG = 16, SMAX = 64, 
 template <> struct GjCfg<4> { static constexpr int RG = 16, CG = 32, SMAX = 128, WPE = 4; };
 template <> struct GjCfg<5> { static constexpr int RG = 16, CG = 32, SMAX = 160, WPE = 2; };
 
+// LDS of the pivot broadcast: row k and column k stored thread-major (the CPT row values a
+// thread needs, then the next thread's), each thread's run padded to an even count, so a
+// thread reads its values with 16-byte loads.
+template <int RG, int CG, int SMAX>
+struct GjBuf {
+  static constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
+  static constexpr int RPTP = (RPT + 1) & ~1, CPTP = (CPT + 1) & ~1;
+  static constexpr int ROW = CG * CPTP, COL = RG * RPTP;   // doubles per buffer
+};
+
 // In-place Gauss-Jordan inverse of an SPD s x s matrix (s <= SMAX) held by the workgroup's
 // RG x CG threads: element (i, j) in a[i / RG][j / CG] of thread (i % RG, j % CG), padding
 // rows / columns set to the identity.  One barrier per pivot.
@@ -600,9 +610,11 @@ template <> struct GjCfg<5> { static constexpr int RG = 16, CG = 32, SMAX = 160,
 // rewriting them after the rank-1 update a <- a - u v^T (u = column k, v = row k / p) are
 // plain masked register moves, with no selects over slots.  The pivot owner broadcasts 1 / p.
 template <int RG, int CG, int SMAX>
-__device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int s, double (&rowb)[2][SMAX],
-                                          double (&colb)[2][SMAX], double (&ipb)[2]) {
-  constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
+__device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int s, double* rowb, double* colb,
+                                          double* ipb) {
+  using Buf = GjBuf<RG, CG, SMAX>;
+  constexpr int RPT = Buf::RPT, CPT = Buf::CPT, RPTP = Buf::RPTP, CPTP = Buf::CPTP;
+  typedef double doublex2 __attribute__((ext_vector_type(2)));
   static_assert(CG % RG == 0 && RG % 2 == 0, "column slot of a pivot must follow from its row slot");
   const int tid = threadIdx.x, ty = tid / CG, tx = tid % CG;
 #pragma unroll
@@ -612,14 +624,16 @@ __device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int
 #pragma unroll 1
     for (int ky = 0; ky < kend; ++ky) {
       const int bf = ky & 1;
+      double* const rb = rowb + bf * Buf::ROW;
+      double* const cb = colb + bf * Buf::COL;
       const int kx = kr * RG - kq * CG + ky;
       if (ty == ky) {
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) rowb[bf][tx + CG * q] = a[kr][q];
+        for (int q = 0; q < CPT; ++q) rb[tx * CPTP + q] = a[kr][q];
       }
       if (tx == kx) {
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) colb[bf][ty + RG * r] = a[r][kq];
+        for (int r = 0; r < RPT; ++r) cb[ty * RPTP + r] = a[r][kq];
         if (ty == ky) {
           double p = a[kr][kq];
           p = p > 1e-300 ? p : 1e-300;
@@ -628,11 +642,19 @@ __device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int
       }
       __syncthreads();
       const double ip = ipb[bf];
-      double v[CPT], u[RPT];
+      double v[CPTP], u[RPTP];
 #pragma unroll
-      for (int q = 0; q < CPT; ++q) v[q] = rowb[bf][tx + CG * q] * ip;
+      for (int q = 0; q < CPTP; q += 2) {
+        const doublex2 t = *(const doublex2*)(rb + tx * CPTP + q);
+        v[q] = t[0] * ip;
+        v[q + 1] = t[1] * ip;
+      }
 #pragma unroll
-      for (int r = 0; r < RPT; ++r) u[r] = colb[bf][ty + RG * r];
+      for (int r = 0; r < RPTP; r += 2) {
+        const doublex2 t = *(const doublex2*)(cb + ty * RPTP + r);
+        u[r] = t[0];
+        u[r + 1] = t[1];
+      }
 #pragma unroll
       for (int r = 0; r < RPT; ++r)
 #pragma unroll
@@ -668,7 +690,9 @@ void newton_gj_kernel(
   constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX, NTH = RG * CG;
   constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
   static_assert(RPT * RG == SMAX && CPT * CG == SMAX, "register grid must tile the matrix");
-  __shared__ double rowb[2][SMAX], colb[2][SMAX], ipb[2];
+  __shared__ __attribute__((aligned(16))) double rowb[2 * GjBuf<RG, CG, SMAX>::ROW];
+  __shared__ __attribute__((aligned(16))) double colb[2 * GjBuf<RG, CG, SMAX>::COL];
+  __shared__ double ipb[2];
   __shared__ double red[2 * (NTH / 64)];
   const MatRef m = find_mat(groups, n_groups, mat_begin, blockIdx.x);
   const int s = (int)m.g.s, ld = (int)m.g.ld;
@@ -737,7 +761,9 @@ template <int WT>
 __global__ __launch_bounds__(GjCfg<WT>::RG * GjCfg<WT>::CG) void spd_gj_kernel(const FhSpdSys* __restrict__ sys) {
   constexpr int RG = GjCfg<WT>::RG, CG = GjCfg<WT>::CG, SMAX = GjCfg<WT>::SMAX;
   constexpr int RPT = SMAX / RG, CPT = SMAX / CG;
-  __shared__ double rowb[2][SMAX], colb[2][SMAX], ipb[2];
+  __shared__ __attribute__((aligned(16))) double rowb[2 * GjBuf<RG, CG, SMAX>::ROW];
+  __shared__ __attribute__((aligned(16))) double colb[2 * GjBuf<RG, CG, SMAX>::COL];
+  __shared__ double ipb[2];
   const FhSpdSys q = sys[blockIdx.x];
   const int s = (int)q.s;
   if (ns_class(s) != WT) return;
