@@ -28,51 +28,50 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+WHAT = {"y": "Y = X C", "that": "T_hat = P^T X", "ttilde": "T_tilde = T_hat M"}
+
+
 def launch_work(eng, phase):
     """Algorithmic work of one single-launch phase: (bound, bytes, flops, description).
 
-    'that:b<bm>' (streaming bf16 kernel, tile height bm) and 'that:<shape>' (tiled kernel)
-    are one grouped launch of T_hat_i = P_i^T X_i over every bin whose chromosome uses that
-    tile: per bin 2 s w N flops; bytes = X_i (w x N) + P_i (s x w) read once, T_hat_i (s x N)
-    written once, at the path's element size.  Padding (cells up to a multiple of 64,
-    positions up to a multiple of 64) is not counted."""
-    if not phase.startswith("that:"):
+    The phases timed launch by launch are the grouped launches of Y_c = X_c C_c, T_hat_i =
+    P_i^T X_i and T_tilde_c = T_hat_c M ('<what>:s<bm>x<bn>' streaming kernel, '<what>:t<shape>'
+    tiled kernel).  Per descriptor batch item of an M x N x K product: 2 M N K flops; bytes =
+    A (M x K) and B (K x N) read once, C (M x N) written once, at the operands' element sizes
+    and stored shapes (cells padded to a multiple of 64, positions not padded)."""
+    from fasthigashi import ops, _lib as L
+    g = eng.launch_groups().get(phase)
+    if g is None:
         return None
-    from fasthigashi import ops
-    tag = phase.split(":")[1]
-    if tag.startswith("b"):
-        member = lambda s: ops.stream_bm(s) == int(tag[1:])
+    if isinstance(g, ops.StreamGemmGroup):
+        ea = eb = 2
     else:
-        member = lambda s: (int(tag) == 5) == (s > 128)
-    xe = 2 if eng.dtype == "bf16" else 4
-    N = eng.N
+        ea = eb = {L.F32: 4, L.BF16: 2, L.FP8: 1}[g.dtype]
     fl = by = 0.0
-    bins = 0
-    for cl in eng.chroms:
-        if not member(cl.s):
-            continue
-        for tl in cl.tiles:
-            w, s = tl.width, cl.s
-            fl += tl.rows * 2.0 * s * w * N
-            by += tl.rows * float(xe) * (w * N + s * w + s * N)
-            bins += tl.rows
-    ridge = PEAK["bf16" if eng.dtype == "bf16" else "fp32"] / PEAK["hbm"]
-    bound = "hbm" if fl / by < ridge else "mfma"
-    return bound, by, fl, "%d bins, %d cells" % (bins, N)
+    items = 0
+    for d in g.descs:
+        ec = 2 if (d.flags & L.GEMM_OUT_BF16) else 4
+        fl += d.batch * 2.0 * d.M * d.N * d.K
+        by += d.batch * float(d.M * d.K * ea + d.K * d.N * eb + d.M * d.N * ec)
+        items += d.batch
+    peak = PEAK["bf16" if eb == 2 else "fp32"]
+    bound = "hbm" if fl / by < peak / PEAK["hbm"] else "mfma"
+    return bound, by, fl, "%s: %d descriptors, %d products" % (WHAT[phase.split(":")[0]], len(g.descs), items)
 
 
 def kernel_name(phase, head):
-    from fasthigashi import ops
     tag = phase.split(":")[1]
-    if tag.startswith("b"):
-        return "gemm_stream_kernel, bf16 operands, %dx128 tiles, global->LDS ring" % int(tag[1:])
-    return "gemm_mfma_kernel, %s operands, %dx%d tiles" % (head, *ops.SHAPES[int(tag)])
+    if tag.startswith("s"):
+        bm, bn = tag[1:].split("x")
+        return "gemm_stream_kernel, bf16 operands, %sx%s tiles, global->LDS ring" % (bm, bn)
+    from fasthigashi import ops
+    return "gemm_mfma_kernel, %s operands, %dx%d tiles" % (head, *ops.SHAPES[int(tag[1:])])
 
 
 def traffic_from_profiles(kernel_tag):
     """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), when present."""
-    path = os.path.join(ROOT, "profiles", "that_bf16_c2_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "stream_gemm_bf16_c2_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -254,7 +253,8 @@ def main():
         pk = PEAK["bf16" if head == "bf16" else "fp32"]
         roof = {"bound": "mfma", "achieved": nflops / sec / 1e12, "peak": pk / 1e12, "unit": "TFLOP/s",
                 "frac": nflops / sec / pk, "traffic": traffic_from_profiles(dominant)}
-    roof.update({"kernel": "T_hat = P^T X (%s): %s" % (dominant, kname), "launch_ms": phases[dominant],
+    roof.update({"kernel": "%s (%s): %s" % (WHAT[dominant.split(":")[0]], dominant, kname),
+                 "launch_ms": phases[dominant],
                  "algorithmic_bytes": nbytes, "algorithmic_flops": nflops, "work": what,
                  "mfma_frac": nflops / sec / PEAK["bf16" if head == "bf16" else "fp32"]})
     cpu = None

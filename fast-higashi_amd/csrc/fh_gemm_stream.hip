@@ -21,9 +21,9 @@
 // the instruction cache.
 //
 // Contract (checked by the host): both operands k-contiguous, leading dims and batch strides
-// multiples of 8 elements, 16-byte aligned bases, and every row readable and zero from K up
-// to the next multiple of 64 (k-steps load whole 64-element chunks).  flags: 0 or
-// FH_GEMM_OUT_BF16; beta and diag must be 0.
+// multiples of 8 elements, 16-byte aligned bases, and either K % 8 == 0 or every row zero
+// from K up to the next multiple of 8 (a chunk that K ends inside is read whole).  flags: 0
+// or FH_GEMM_OUT_BF16; beta and diag must be 0.
 #include "fh_common.h"
 
 namespace {
@@ -123,6 +123,7 @@ struct LoadRef {
   const uint8_t* p[D];
   uint32_t kstep;    // 128 (bytes of 64 bf16), 0 for a tile past the end
   int nk;            // k-steps
+  int nv_last;       // 8-element chunks of the last k-step inside K (8: all)
   bool live;
 };
 
@@ -135,6 +136,7 @@ __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int&
     for (int q = 0; q < C::D; ++q) r.p[q] = (const uint8_t*)g_zero16;
     r.kstep = 0;
     r.nk = 1;
+    r.nv_last = 8;
     r.live = false;
     return r;
   }
@@ -156,6 +158,7 @@ __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int&
   }
   r.kstep = 128;
   r.nk = (int)((d.K + 63) / 64);
+  r.nv_last = (int)((d.K - 64 * (int64_t)(r.nk - 1) + 7) / 8);
   r.live = true;
   return r;
 }
@@ -169,12 +172,21 @@ __device__ __forceinline__ void dma16(const void* src, uint32_t lds) {
                : "memory", "m0");
 }
 
-// issue the loads of k-step kk of tile r into the stage at LDS address st
+// issue the loads of k-step kk of tile r into the stage at LDS address st.  A lane's chunk
+// (l % 8) ^ (l / 8) is the same for all its loads; on the last k-step, lanes whose chunk lies
+// past K load the zero block, so the padding of K up to a multiple of 64 is never fetched.
 template <class C>
-__device__ __forceinline__ void issue_stage(const LoadRef<C::D>& r, int kk, uint32_t st, int wave) {
+__device__ __forceinline__ void issue_stage(const LoadRef<C::D>& r, int kk, uint32_t st, int wave, int lane) {
   const uint32_t kb = (uint32_t)kk * r.kstep;
+  if (kk == r.nk - 1 && r.nv_last < 8) {           // uniform
+    const bool in = ((lane & 7) ^ (lane >> 3)) < r.nv_last;
 #pragma unroll
-  for (int q = 0; q < C::D; ++q) dma16(r.p[q] + kb, st + (wave + C::NW * q) * 1024);
+    for (int q = 0; q < C::D; ++q)
+      dma16(in ? (const void*)(r.p[q] + kb) : (const void*)g_zero16, st + (wave + C::NW * q) * 1024);
+  } else {
+#pragma unroll
+    for (int q = 0; q < C::D; ++q) dma16(r.p[q] + kb, st + (wave + C::NW * q) * 1024);
+  }
 }
 
 // s_waitcnt vmcnt(n) then the workgroup barrier; a compiler memory fence, so no LDS access
@@ -324,7 +336,7 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
     for (int j = 0; j < WN_T; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   auto issue = [&](int stage) {
-    issue_stage<C>(lr, lk, ring_lds + stage * C::STAGE_BYTES, wave);
+    issue_stage<C>(lr, lk, ring_lds + stage * C::STAGE_BYTES, wave, lane);
     if (++lk >= lr.nk) {
       lk = 0;
       lt += stride;

@@ -159,9 +159,9 @@ class StreamGemmGroup:
         assert len(descs) > 0
         for d in descs:
             assert d.sAk == 1 and d.sBk == 1, "streaming GEMM needs k-contiguous A and B"
-            # k-steps load whole 64-element chunks: rows readable (and zero) up to K rounded to 64
+            # 16-byte chunks along k: rows readable (and zero) up to K rounded to 8
             for ld, bs, p, nm in ((d.sAm, d.sAb, d.A, "A"), (d.sBn, d.sBb, d.B, "B")):
-                assert ld % 8 == 0 and ld >= -(-d.K // 64) * 64, "%s leading dim %d (K %d)" % (nm, ld, d.K)
+                assert ld % 8 == 0 and ld >= -(-d.K // 8) * 8, "%s leading dim %d (K %d)" % (nm, ld, d.K)
                 assert d.batch == 1 or bs % 8 == 0, "%s batch stride %d not a multiple of 8" % (nm, bs)
                 assert p % 16 == 0, "%s pointer not 16-byte aligned" % nm
             assert d.beta == 0.0 and d.diag == 0.0 and (d.flags & ~L.GEMM_OUT_BF16) == 0, \

@@ -269,9 +269,9 @@ def test_polar_newton_matches_eigh(cuda, log_kappa):
 
 def _stream_case(cuda, gen, M, N, K, batch, out16):
     """operands of T_hat = P^T X in the streaming layout: A (batch, sp, wp) and B (batch, N, wp),
-    both k-contiguous with the k-padding (to a multiple of 64) zero; C (batch, sp, N)."""
+    both k-contiguous with the k-padding (to a multiple of 8) zero; C (batch, sp, N)."""
     from fasthigashi import ops, _lib as L
-    wp = -(-K // 64) * 64
+    wp = -(-K // 8) * 8
     sp = -(-M // 4) * 4
     A = torch.zeros(batch, sp, wp)
     B = torch.zeros(batch, N, wp)
