@@ -58,13 +58,31 @@ __global__ void copy2d_kernel(const void* src, int sdt, int64_t srs, int64_t scs
 }
 
 // Row scaling of per-bin blocks:  Y[r][j] *= A[bin0 + r / w][j]   (j < s), one tile per blockIdx.y.
+// One wave per row (the bin index once per row); with ldy, lda multiples of 4 and 16-byte
+// aligned bases each lane scales four columns with 16-byte accesses, else one column at a time.
 struct RowScale { float* Y; const float* A; int64_t nrows, w, s, ldy, lda; };
-__global__ void scale_rows_kernel(const RowScale* __restrict__ tab) {
+__global__ __launch_bounds__(NT) void scale_rows_kernel(const RowScale* __restrict__ tab) {
   const RowScale t = tab[blockIdx.y];
-  const int64_t n = t.nrows * t.s;
-  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-    int64_t r = i / t.s, j = i % t.s;
-    t.Y[r * t.ldy + j] *= t.A[(r / t.w) * t.lda + j];
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int64_t nw = (int64_t)gridDim.x * (NT / 64);
+  const bool vec = ((t.ldy | t.lda) & 3) == 0 && ((((uintptr_t)t.Y) | ((uintptr_t)t.A)) & 15) == 0;
+  for (int64_t r = wave; r < t.nrows; r += nw) {
+    float* y = t.Y + r * t.ldy;
+    const float* a = t.A + (r / t.w) * t.lda;
+    if (vec) {
+      for (int64_t j = 4 * lane; j < t.s; j += 256) {
+        if (j + 4 <= t.s) {
+          floatx4 v = *(floatx4*)(y + j);
+          v *= *(const floatx4*)(a + j);
+          *(floatx4*)(y + j) = v;
+        } else {
+          for (int64_t e = j; e < t.s; ++e) y[e] *= a[e];
+        }
+      }
+    } else {
+      for (int64_t j = lane; j < t.s; j += 64) y[j] *= a[j];
+    }
   }
 }
 
@@ -233,8 +251,9 @@ int fh_copy2d(const void* src, int sdt, int64_t srs, int64_t scs, void* dst, int
 
 int fh_scale_rows(const FhRowScale* tab_dev, int n_tiles, int64_t max_elems, hipStream_t st) {
   if (n_tiles <= 0) return FH_OK;
-  unsigned gx = grid_for(max_elems);
-  if (gx > 4096) gx = 4096;
+  // max_elems bounds rows x s of a tile; one wave per row, NT / 64 rows per workgroup
+  int64_t gx64 = (max_elems + 4 * NT - 1) / (4 * NT);
+  unsigned gx = (unsigned)(gx64 < 1 ? 1 : gx64 > 1024 ? 1024 : gx64);
   hipLaunchKernelGGL(scale_rows_kernel, dim3(gx, n_tiles), dim3(NT), 0, st, (const RowScale*)tab_dev);
   FH_CHECK_LAUNCH();
   return FH_OK;
