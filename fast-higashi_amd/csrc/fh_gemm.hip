@@ -171,9 +171,11 @@ __device__ __forceinline__ void mfma_step(const u32x4 (&a)[2], const u32x4 (&b)[
 }
 
 // WM_T x WN_T MFMA tiles (16x16) per wave, 2 x 2 waves per workgroup.
-template <typename T, bool AK, bool BK_, int WM_T, int WN_T>
+// RESID: the epilogue stores nothing; C holds fp32 data X and the workgroup writes
+// part[blockIdx.x] = sum over its tile of (X - alpha * A B)^2, accumulated in fp64.
+template <typename T, bool AK, bool BK_, int WM_T, int WN_T, bool RESID = false>
 __global__ __launch_bounds__(NT) void gemm_mfma_kernel(const FhGemmDesc* __restrict__ descs,
-                                                       int n_desc, FhGemmDesc single) {
+                                                       int n_desc, FhGemmDesc single, double* part) {
   constexpr int BM = 32 * WM_T, BN = 32 * WN_T, BK = Elem<T>::BK;
   __shared__ uint32_t As[BM * LDS_ROW];
   __shared__ uint32_t Bs[BN * LDS_ROW];
@@ -182,11 +184,13 @@ __global__ __launch_bounds__(NT) void gemm_mfma_kernel(const FhGemmDesc* __restr
   const FhGemmDesc* dp = &single;
   if (descs != nullptr) dp = &descs[find_desc(descs, n_desc, t)];
   const FhGemmDesc d = *dp;
-  if (d.skip && *d.skip) return;
   const int ksplit = d.flags >> 8 ? (d.flags >> 8) : 1;
   int64_t local = t - d.tile_begin;
   const int64_t per_batch = (int64_t)d.tiles_m * d.tiles_n * ksplit;
-  if (local >= per_batch * d.batch) return;
+  if ((d.skip && *d.skip) || local >= per_batch * d.batch) {
+    if (RESID && threadIdx.x == 0) part[t] = 0.0;
+    return;
+  }
   const int64_t b = local / per_batch;
   local -= b * per_batch;
   const int ks = (int)(local % ksplit);
@@ -254,6 +258,30 @@ __global__ __launch_bounds__(NT) void gemm_mfma_kernel(const FhGemmDesc* __restr
   float scale = (float)d.alpha;
   if (d.scale_a) scale *= d.scale_a[b];
   if (d.scale_b) scale *= d.scale_b[b];
+  if constexpr (RESID) {
+    const float* X = (const float*)d.C + b * d.sCb;
+    double ss = 0.0;
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+      for (int j = 0; j < WN_T; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int64_t row = m0 + wm * WM_T * 16 + i * 16 + (lane >> 4) * 4 + r;
+          int64_t col = n0 + wn * WN_T * 16 + j * 16 + (lane & 15);
+          if (row < d.M && col < d.N) {
+            float e = X[row * d.sCm + col * d.sCn] - scale * acc[i][j][r];
+            ss = fma((double)e, (double)e, ss);
+          }
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+    __shared__ double red[NT / 64];
+    if (lane == 0) red[wave] = ss;
+    __syncthreads();
+    if (tid == 0) part[t] = (red[0] + red[1]) + (red[2] + red[3]);
+    return;
+  }
   const int64_t coff = b * d.sCb + ((d.flags & FH_GEMM_SPLIT_PARTIAL) ? ks * d.sCk : 0);
   float* C = (float*)d.C + coff;
   uint16_t* Ch = (uint16_t*)d.C + coff;
@@ -501,10 +529,11 @@ template <typename T, int WM, int WN>
 int launch_typed(bool ak, bool bk, const FhGemmDesc* dd, int n, const FhGemmDesc& single,
                  int64_t tiles, hipStream_t st) {
   dim3 grid((unsigned)tiles), block(NT);
-  if (ak && bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, true, true, WM, WN>), grid, block, 0, st, dd, n, single);
-  else if (ak && !bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, true, false, WM, WN>), grid, block, 0, st, dd, n, single);
-  else if (!ak && bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, false, true, WM, WN>), grid, block, 0, st, dd, n, single);
-  else hipLaunchKernelGGL((gemm_mfma_kernel<T, false, false, WM, WN>), grid, block, 0, st, dd, n, single);
+  double* np_ = nullptr;
+  if (ak && bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, true, true, WM, WN>), grid, block, 0, st, dd, n, single, np_);
+  else if (ak && !bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, true, false, WM, WN>), grid, block, 0, st, dd, n, single, np_);
+  else if (!ak && bk) hipLaunchKernelGGL((gemm_mfma_kernel<T, false, true, WM, WN>), grid, block, 0, st, dd, n, single, np_);
+  else hipLaunchKernelGGL((gemm_mfma_kernel<T, false, false, WM, WN>), grid, block, 0, st, dd, n, single, np_);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }
@@ -570,6 +599,32 @@ extern "C" int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhG
     case 5: hipLaunchKernelGGL((gemm_f64_kernel<double, float, double>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
     case 6: hipLaunchKernelGGL((gemm_f64_kernel<double, double, float>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
     default: hipLaunchKernelGGL((gemm_f64_kernel<double, double, double>), grid, block, 0, stream, descs_dev, n_desc, s1); break;
+  }
+  FH_CHECK_LAUNCH();
+  return FH_OK;
+}
+
+// Fused residual of the reconstruction loss: per workgroup tile the fp64 sum of
+// (X - alpha * A B)^2, A k-contiguous fp32 (Q = P B diag(A_i)), B n-contiguous fp32 (C^T),
+// X = the descriptor's C (fp32, read only).  part[t] for every tile t < total_tiles.
+extern "C" int fh_loss_fwd(int shape, const FhGemmDesc* descs_dev, int n_desc, const FhGemmDesc* single,
+                           int64_t total_tiles, double* part, hipStream_t stream) {
+  if (total_tiles <= 0) return FH_OK;
+  FhGemmDesc s1{};
+  if (single) {
+    s1 = *single;
+    FH_REQUIRE((s1.flags & 0xff) == 0 && (s1.flags >> 8) <= 1, "loss_fwd: flags must be 0 (no split, no atomics)");
+    FH_REQUIRE(s1.sAk == 1 && s1.sBn == 1, "loss_fwd: A must be k-contiguous and B n-contiguous");
+  }
+  FH_REQUIRE(descs_dev != nullptr || single != nullptr, "loss_fwd: no descriptor");
+  FH_REQUIRE(part != nullptr, "loss_fwd: null partial-sum buffer");
+  FH_REQUIRE(total_tiles < (int64_t)1 << 31, "loss_fwd: too many tiles");
+  dim3 grid((unsigned)total_tiles), block(NT);
+  switch (shape) {
+    case 0: hipLaunchKernelGGL((gemm_mfma_kernel<float, true, false, 4, 4, true>), grid, block, 0, stream, descs_dev, n_desc, s1, part); break;
+    case 1: hipLaunchKernelGGL((gemm_mfma_kernel<float, true, false, 4, 2, true>), grid, block, 0, stream, descs_dev, n_desc, s1, part); break;
+    case 6: hipLaunchKernelGGL((gemm_mfma_kernel<float, true, false, 4, 8, true>), grid, block, 0, stream, descs_dev, n_desc, s1, part); break;
+    default: fh_set_error("loss_fwd: tile shape %d not built (0, 1 or 6)", shape); return FH_ERR_ARG;
   }
   FH_CHECK_LAUNCH();
   return FH_OK;

@@ -71,6 +71,15 @@ class CpChrom(Structure):
                 ("mdsplit", c_int64)]
 
 
+class LossChrom(Structure):
+    _fields_ = [("H", c_void_p), ("A", c_void_p), ("B", c_void_p), ("gA", c_void_p), ("gBpart", c_void_p),
+                ("n", c_int64), ("s", c_int64), ("sp", c_int64), ("chunk_stride", c_int64), ("blk_begin", c_int64),
+                ("blk_a", c_int32), ("n_chunks", c_int32), ("blk_b", c_int32), ("pad", c_int32)]
+
+
+LOSS_CHUNK = 32     # FH_LOSS_CHUNK: bins per partial sum of the B gradient
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "fh_last_error": (c_char_p, []),
@@ -124,6 +133,9 @@ _SIGS = {
                             c_void_p]),
     "fh_colsum_add": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64,
                               c_void_p]),
+    "fh_loss_fwd": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
+    "fh_loss_prepare": (c_int64, [c_void_p, c_int]),
+    "fh_loss_bwd": (c_int, [c_void_p, c_int, c_int64, c_float, c_void_p]),
 }
 
 _OPTIONAL = set()

@@ -1,0 +1,285 @@
+"""Differentiable PARAFAC2 reconstruction loss over a SweepEngine's resident data (gfx950).
+
+    L(A, B, D, M) = sum_c sum_i || X_i - P_i B_c diag(A_ci) C_c^T ||^2,   C_c = M D_c
+
+with the data X and the per-bin projections P of the engine held fixed (P is the closed-form
+Procrustes solution of the sweep, parafac2_intergrative.py:396-417; it stays out of the
+gradient).  The reference evaluates this loss only through its norm decomposition
+(parafac2_intergrative.py:667-668) and never differentiates it; here it is a
+torch.autograd.Function so the ALS factors can be refined by gradient steps.
+
+Forward (no slice of the reconstruction is stored):
+    C^T_c = (M D_c)^T                         fp32 MFMA GEMM
+    Q_c   = P_c B_c, rows of bin i * A_i       fp32 MFMA GEMM + row scaling
+    L     = sum (X - Q C^T)^2                  fh_loss_fwd: the GEMM Q C^T with a residual
+                                               epilogue, fp64 per-tile partial sums
+
+Backward (recomputes C^T and Q from the saved factors; gradient = MTTKRP minus factor Gram):
+    F_c   = X_c C_c - Q_c (C_c^T C_c)          X read once (fp32 GEMM), Gram in fp64
+    H_i   = P_i^T F_i                          per-bin batched GEMM
+    dA, dB from H                              fh_loss_bwd (fp64 accumulation, fixed order)
+    dC_c  = 2 C_c (Q_c^T Q_c) - 2 X_c^T Q_c    X read a second time (k-split), Gram in fp64
+    dD_c  = M^T dC_c,  dM = sum_c dC_c D_c^T
+
+With P_i^T P_i = I these are the familiar CP gradients, e.g. dA = 2 (A (B^T B o C^T C) - MTTKRP);
+the forms above hold for any P.
+
+Factors use the engine's packed layouts: A_all (sum n_c sp_c), B_all (sum s_c sp_c),
+D_all (R x Dld), M (Npad x R); padding entries get zero gradient and stay zero under Adam.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import ops
+from .engine import MultiGemm, SweepEngine, shape_for_n
+from .ops import make_desc, rup
+
+F32, F64 = L.F32, L.F64
+
+
+class LossPlan:
+    """Buffers and grouped launches of the loss and its gradient for one engine (fp32 data)."""
+
+    FWD_SHAPE = 0            # 128 x 128 tiles of the residual GEMM
+
+    def __init__(self, eng: SweepEngine, zx_target_tiles: int = 2048, max_partial_bytes: int = 4 << 30):
+        if eng.dtype != "fp32":
+            raise ValueError("the differentiable loss needs the fp32 data of an fp32 engine")
+        self.eng = eng
+        dev = self.device = eng.device
+        R, Np, Dld, N = eng.R, eng.Npad, eng.Dld, eng.N
+        ch = eng.chroms
+        self.K = len(ch)
+        z = lambda *shape, dt=torch.float32: torch.zeros(*shape, dtype=dt, device=dev)
+        f4 = lambda t, o=0: t.data_ptr() + 4 * o
+        # factor copies the launches read (inputs are copied in; nothing else is kept)
+        self.A = z(eng.A_all.numel())
+        self.B = z(eng.B_all.numel())
+        self.D = z(R, Dld)
+        self.M = z(Np, R)
+        # work buffers
+        self.Ct = z(Dld, Np)
+        self.Q = z(eng.Y.numel())
+        self.F = z(eng.Y.numel())
+        self.hoff = []
+        h = 0
+        for cl in ch:
+            self.hoff.append(h)
+            h += cl.n * cl.s * cl.sp
+        self.H = z(h)
+        nb = eng.B_all.numel()           # sum s_c sp_c; B_all layout for every s x sp block
+        self.Gc = z(nb)
+        self.Gq = z(nb, dt=torch.float64)
+        self.GC = z(Np, Dld)             # dL/dC_c at columns [doff, doff + s)
+        self.gA = z(eng.A_all.numel())
+        self.gB = z(nb)
+        self.gD = z(R, Dld)
+        self.gM = z(Np, R)
+        self.loss_c = z(self.K, dt=torch.float64)
+        self.loss = z(1, dt=torch.float64)
+        Xp = eng.X.data_ptr()
+        # (1) C^T = (M D)^T
+        d = [make_desc(self.M.data_ptr(), f4(self.D, cl.doff), f4(self.Ct, cl.doff * Np), Np, cl.s, R, R, 1,
+                       Dld, 1, 1, Np) for cl in ch]
+        self.g_ct = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
+        # (2) Q = P B, then rows of bin i times A_i
+        d = [make_desc(f4(eng.P, cl.yoff), f4(self.B, cl.boff), f4(self.Q, cl.yoff), cl.W, cl.s, cl.s, cl.sp, 1,
+                       cl.sp, 1, cl.sp, 1) for cl in ch]
+        self.g_q = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
+        rs = []
+        for c, cl in enumerate(ch):
+            for t, tl in enumerate(cl.tiles):
+                q = L.RowScale()
+                q.Y = f4(self.Q, cl.yoff + cl.trow[t] * cl.sp)
+                q.A = f4(self.A, cl.aoff + tl.start * cl.sp)
+                q.nrows, q.w, q.s, q.ldy, q.lda = tl.rows * tl.width, tl.width, cl.s, cl.sp, cl.sp
+                rs.append(q)
+        self.rs_tab = ops._upload((L.RowScale * len(rs))(*rs), dev)
+        self.rs_n = len(rs)
+        self.rs_max = max(q.nrows * q.s for q in rs)
+        # (3) forward: residual GEMM over X, one fp64 partial per tile
+        d = [make_desc(f4(self.Q, cl.yoff), f4(self.Ct, cl.doff * Np), Xp + 4 * cl.xrow * Np, cl.W, N, cl.s,
+                       cl.sp, 1, Np, 1, Np, 1) for cl in ch]
+        bm, bn = ops.SHAPES[self.FWD_SHAPE]
+        arr = (L.GemmDesc * len(d))(*d)
+        self.fwd_total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(d), bm, bn)
+        self.fwd_n = len(d)
+        self.fwd_tab = ops._upload(arr, dev)
+        self.part = z(self.fwd_total, dt=torch.float64)
+        self.fwd_seg = torch.tensor([q.tile_begin for q in arr] + [self.fwd_total], dtype=torch.int64, device=dev)
+        self.all_seg = torch.tensor([0, self.K], dtype=torch.int64, device=dev)
+        # (4) F = X C
+        d = [make_desc(Xp + 4 * cl.xrow * Np, f4(self.Ct, cl.doff * Np), f4(self.F, cl.yoff), cl.W, cl.s, Np,
+                       Np, 1, 1, Np, cl.sp, 1) for cl in ch]
+        self.g_y = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
+        # (5) Gc = C^T C (fp64 accumulate, fp32 out)
+        d = [make_desc(f4(self.Ct, cl.doff * Np), f4(self.Ct, cl.doff * Np), f4(self.Gc, cl.boff), cl.s, cl.s, Np,
+                       Np, 1, 1, Np, cl.sp, 1) for cl in ch]
+        self.g_gc = ops.Gemm64Group(d, F32, F32, F32, dev)
+        # (6) F -= Q Gc
+        d = [make_desc(f4(self.Q, cl.yoff), f4(self.Gc, cl.boff), f4(self.F, cl.yoff), cl.W, cl.s, cl.s, cl.sp, 1,
+                       cl.sp, 1, cl.sp, 1, alpha=-1.0, beta=1.0) for cl in ch]
+        self.g_f = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
+        # (7) H_i = P_i^T F_i per bin
+        d = []
+        for c, cl in enumerate(ch):
+            for t, tl in enumerate(cl.tiles):
+                o = cl.yoff + cl.trow[t] * cl.sp
+                d.append(make_desc(f4(eng.P, o), f4(self.F, o), f4(self.H, self.hoff[c] + tl.start * cl.s * cl.sp),
+                                   cl.s, cl.s, tl.width, 1, cl.sp, cl.sp, 1, cl.sp, 1, tl.rows,
+                                   tl.width * cl.sp, tl.width * cl.sp, cl.s * cl.sp))
+        self.g_h = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
+        # (8) dA, dB partials from H
+        chunk_stride = nb
+        n_chunks = max(-(-cl.n // L.LOSS_CHUNK) for cl in ch)
+        self.n_chunks = n_chunks
+        self.gBpart = z(n_chunks * nb)
+        tab = []
+        for c, cl in enumerate(ch):
+            q = L.LossChrom()
+            q.H, q.A, q.B = f4(self.H, self.hoff[c]), f4(self.A, cl.aoff), f4(self.B, cl.boff)
+            q.gA, q.gBpart = f4(self.gA, cl.aoff), f4(self.gBpart, cl.boff)
+            q.n, q.s, q.sp, q.chunk_stride = cl.n, cl.s, cl.sp, chunk_stride
+            tab.append(q)
+        arr = (L.LossChrom * len(tab))(*tab)
+        self.bwd_total = L.lib().fh_loss_prepare(ctypes.cast(arr, ctypes.c_void_p), len(tab))
+        self.bwd_tab = ops._upload(arr, dev)
+        # (9) X^T Q, k-split partials over the positions (X read a second time)
+        shp = [shape_for_n(cl.s) for cl in ch]
+        tiles = sum(-(-N // ops.SHAPES[k][0]) * -(-cl.s // ops.SHAPES[k][1]) for k, cl in zip(shp, ch))
+        ksz = max(1, min(64, -(-zx_target_tiles // tiles), min(cl.W for cl in ch) // 1024 or 1))
+        ksz = max(1, min(ksz, max_partial_bytes // (4 * Np * Dld)))
+        self.ksz = ksz
+        self.Zpart = z(ksz * Np * Dld)
+        d = [make_desc(Xp + 4 * cl.xrow * Np, f4(self.Q, cl.yoff), f4(self.Zpart, cl.doff), N, cl.s, cl.W, 1, Np,
+                       cl.sp, 1, Dld, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=ksz, sCk=Np * Dld) for cl in ch]
+        self.g_zx = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
+        # (10) Gq = Q^T Q (fp64), k-split partials
+        t64 = sum(-(-cl.s // 64) ** 2 for cl in ch)
+        ksq = max(1, min(64, -(-1024 // t64), min(cl.W for cl in ch) // 512 or 1))
+        self.ksq = ksq
+        self.Gqpart = z(ksq * nb, dt=torch.float64)
+        d = [make_desc(f4(self.Q, cl.yoff), f4(self.Q, cl.yoff), self.Gqpart.data_ptr() + 8 * cl.boff, cl.s, cl.s,
+                       cl.W, 1, cl.sp, cl.sp, 1, cl.sp, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=ksq, sCk=nb)
+             for cl in ch]
+        self.g_gq = ops.Gemm64Group(d, F32, F32, F64, dev)
+        # (11) dC = 2 C Gq - 2 X^T Q  (in place over the reduced X^T Q)
+        d = [make_desc(f4(self.Ct, cl.doff * Np), self.Gq.data_ptr() + 8 * cl.boff, f4(self.GC, cl.doff), N, cl.s,
+                       cl.s, 1, Np, cl.sp, 1, Dld, 1, alpha=2.0, beta=-2.0) for cl in ch]
+        self.g_dc = ops.Gemm64Group(d, F32, F64, F32, dev)
+        # (12) dD = M^T dC (all chromosomes: R x Dld), (13) dM = dC D^T
+        self.g_dd = ops.GemmGroup([make_desc(self.M.data_ptr(), self.GC.data_ptr(), self.gD.data_ptr(), R, Dld, N,
+                                             1, R, Dld, 1, Dld, 1)], F32, shape_for_n(Dld), dev)
+        self.g_dm = ops.GemmGroup([make_desc(self.GC.data_ptr(), self.D.data_ptr(), self.gM.data_ptr(), N, R, Dld,
+                                             Dld, 1, 1, Dld, R, 1)], F32, shape_for_n(R), dev)
+
+    # ------------------------------------------------------------------ pieces
+    def load(self, A, B, D, M):
+        """Copy factor tensors (engine layouts) into the buffers the launches read."""
+        for dst, src in ((self.A, A), (self.B, B), (self.D, D), (self.M, M)):
+            if src.shape != dst.shape or src.dtype != torch.float32 or src.device != dst.device:
+                raise ValueError("factor of shape %s / %s / %s, expected %s float32 on %s"
+                                 % (tuple(src.shape), src.dtype, src.device, tuple(dst.shape), dst.device))
+            dst.copy_(src.detach())
+
+    def _q(self, stream=None):
+        self.g_ct.run(stream)
+        self.g_q.run(stream)
+        L.call("fh_scale_rows", self.rs_tab.data_ptr(), self.rs_n, self.rs_max, ops.stream_handle(stream))
+
+    def forward(self, stream=None) -> torch.Tensor:
+        """Loss of the loaded factors: per chromosome into self.loss_c, the total into self.loss."""
+        st = ops.stream_handle(stream)
+        self._q(stream)
+        L.call("fh_loss_fwd", self.FWD_SHAPE, self.fwd_tab.data_ptr(), self.fwd_n, None, self.fwd_total,
+               self.part.data_ptr(), st)
+        L.call("fh_segment_sum", self.part.data_ptr(), self.fwd_seg.data_ptr(), self.K, self.loss_c.data_ptr(), 0, st)
+        L.call("fh_segment_sum", self.loss_c.data_ptr(), self.all_seg.data_ptr(), 1, self.loss.data_ptr(), 0, st)
+        return self.loss
+
+    def backward(self, stream=None):
+        """Gradients of the loaded factors into gA, gB, gD, gM (recomputes C^T and Q)."""
+        st = ops.stream_handle(stream)
+        nb = self.B.numel()
+        self._q(stream)
+        self.g_y.run(stream)
+        self.g_gc.run(stream)
+        self.g_f.run(stream)
+        self.g_h.run(stream)
+        L.call("fh_loss_bwd", self.bwd_tab.data_ptr(), self.K, self.bwd_total, -2.0, st)
+        L.call("fh_batch_sum", self.gBpart.data_ptr(), F32, self.n_chunks, nb, nb, self.gB.data_ptr(), st)
+        self.g_zx.run(stream)
+        Np, Dld = self.eng.Npad, self.eng.Dld
+        L.call("fh_batch_sum", self.Zpart.data_ptr(), F32, self.ksz, Np * Dld, Np * Dld, self.GC.data_ptr(), st)
+        self.g_gq.run(stream)
+        L.call("fh_batch_sum", self.Gqpart.data_ptr(), F64, self.ksq, nb, nb, self.Gq.data_ptr(), st)
+        self.g_dc.run(stream)
+        self.g_dd.run(stream)
+        self.g_dm.run(stream)
+        return self.gA, self.gB, self.gD, self.gM
+
+    # ------------------------------------------------------------------ factors
+    def params_from_engine(self) -> List[torch.Tensor]:
+        """Leaf copies (requires_grad) of the engine's current A, B, D, M."""
+        e = self.eng
+        return [t.detach().clone().requires_grad_(True) for t in (e.A_all, e.B_all, e.D_all, e.M)]
+
+    def write_back(self, params: Sequence[torch.Tensor]):
+        """Copy refined factors into the engine (later sweeps / transform start from them)."""
+        e = self.eng
+        with torch.no_grad():
+            for dst, src in zip((e.A_all, e.B_all, e.D_all, e.M), params):
+                dst.copy_(src)
+        e._core_norms(e.core_cur)
+
+
+class _Parafac2Loss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, plan: LossPlan, A, B, D, M):
+        plan.load(A, B, D, M)
+        out = plan.forward().clone().reshape(())
+        ctx.plan = plan
+        ctx.save_for_backward(A, B, D, M)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        plan = ctx.plan
+        A, B, D, M = ctx.saved_tensors
+        plan.load(A, B, D, M)
+        grads = plan.backward()
+        gs = g.to(torch.float32)
+        return (None,) + tuple(t * gs for t in grads)
+
+
+def parafac2_loss(plan: LossPlan, A: torch.Tensor, B: torch.Tensor, D: torch.Tensor, M: torch.Tensor):
+    """sum_c sum_i ||X_i - P_i B_c diag(A_ci) (M D_c)^T||^2 (float64 scalar, differentiable in the
+    float32 factors A_all, B_all, D_all, M of the engine's layouts)."""
+    return _Parafac2Loss.apply(plan, A, B, D, M)
+
+
+def refine_adam(plan: LossPlan, steps: int = 200, lr: float = 1e-3, params: Optional[List[torch.Tensor]] = None,
+                write_back: bool = False):
+    """Adam on (A, B, D, M) from the engine's factors; returns (params, loss per step as float64 numpy,
+    final loss).  Each step recomputes the projected slices; nothing of the forward is stored."""
+    params = plan.params_from_engine() if params is None else params
+    opt = torch.optim.Adam(params, lr=lr)
+    hist = torch.zeros(steps, dtype=torch.float64, device=plan.device)
+    for k in range(steps):
+        opt.zero_grad(set_to_none=True)
+        loss = parafac2_loss(plan, *params)
+        loss.backward()
+        opt.step()
+        hist[k] = loss.detach()
+    with torch.no_grad():
+        plan.load(*params)
+        final = float(plan.forward().item())
+    if write_back:
+        plan.write_back(params)
+    return params, hist.cpu().numpy(), final

@@ -275,6 +275,34 @@ int fh_transpose_grouped(const FhTransDesc* descs_dev, int n_desc, int64_t total
 /* X[b][i][i] = v (add = 0) or += v (add = 1): torch.eye / diagonal add, partial_rwr.py:113, 118 */
 int fh_diag(float* X, int64_t nb, int64_t stride, int64_t ld, int64_t n, float v, int add, fh_stream_t stream);
 
+/* ------------------------------------------------------------------ reconstruction loss
+ * L = sum_c sum_i ||X_i - P_i B_c diag(A_ci) C_c^T||^2, C_c = M D_c, P held fixed.  No
+ * counterpart: the reference evaluates this loss only through its norm decomposition
+ * (parafac2_intergrative.py:667-668) and never differentiates it.  The forward is the fp32
+ * MFMA GEMM Q C^T (Q = P B diag(A), k-contiguous; C^T n-contiguous) whose epilogue reads X
+ * (the descriptor's C, fp32) and writes part[t] = sum over tile t of (X - alpha Q C^T)^2 in
+ * fp64 for every tile t < total_tiles; flags 0, no k-split; shape 0, 1 or 6. */
+int fh_loss_fwd(int shape, const FhGemmDesc* descs_dev, int n_desc, const FhGemmDesc* single_host,
+                int64_t total_tiles, double* part, fh_stream_t stream);
+/* Backward for A and B from H_i = P_i^T (X_i C - Q_i C^T C) (n x s x sp per chromosome):
+ * gA[i, r] = scale sum_p B[p, r] H_i[p, r] (all r < sp; 0 for r >= s) and, per chunk k of
+ * FH_LOSS_CHUNK bins, gBpart[k * chunk_stride + p * sp + r] = scale sum_{i in chunk k} H_i[p, r] A[i, r]
+ * (summed over chunks afterwards, e.g. by fh_batch_sum).  scale = -2 dL-upstream.  No counterpart. */
+#define FH_LOSS_CHUNK 32
+typedef struct FhLossChrom {
+  const float* H;      /* n x s x sp */
+  const float* A;      /* n x sp */
+  const float* B;      /* s x sp */
+  float* gA;           /* n x sp */
+  float* gBpart;       /* n_chunks blocks of s x sp, chunk_stride elements apart */
+  int64_t n, s, sp, chunk_stride;
+  int64_t blk_begin;   /* filled by fh_loss_prepare */
+  int32_t blk_a, n_chunks, blk_b, pad;
+} FhLossChrom;
+/* host: fill blk_begin / blk_a / n_chunks / blk_b of n entries; returns the total block count */
+int64_t fh_loss_prepare(FhLossChrom* tab_host, int n);
+int fh_loss_bwd(const FhLossChrom* tab_dev, int n_tab, int64_t total_blocks, float scale, fh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,11 @@ Pinning status of each function (what independent check fixes it):
 * sweep / fit / transform: every contraction checked against a float64
   brute-force einsum, the error decomposition checked against the directly
   evaluated PARAFAC2 loss (tests/test_oracle_sweep.py).
+* parafac2_loss / parafac2_loss_grad: central finite differences of the loss for every
+  factor entry of small random cases (non-orthonormal P too), zero loss and zero
+  gradient at an exactly representable model, and for orthonormal P and M the
+  reference's norm decomposition ||X||^2 - 2 <P^T X M, [[A, B, D]]> + core norm
+  (parafac2_intergrative.py:621-632,667-668) (tests/test_oracle_loss.py).
 * init_params: parity unpinned for the random draws (the reference never seeds,
   parafac2_intergrative.py:71-80,257-258); the oracle and the product share an
   injected init instead.

@@ -529,3 +529,35 @@ def parafac2_loss(data: List[ChromData], F: Factors, projections, dt=np.float64)
             rec = P @ core
             tot += float(np.square(cd.X[t].astype(dt) - rec).sum())
     return tot
+
+
+def parafac2_loss_grad(data: List[ChromData], F: Factors, projections, dt=np.float64):
+    """Gradient of parafac2_loss in (A_c, B_c, D_c, M) with P held fixed, from the residuals
+    E_i = X_i - P_i W_i C^T (W_i = B diag(A_i), C = M D):
+        dL/dQ_i = -2 E_i C (Q_i = P_i W_i),  dL/dA_i[r] = sum_w (dL/dQ_i)[w, r] (P_i B)[w, r],
+        dL/dB = sum_i P_i^T (dL/dQ_i) diag(A_i),  dL/dC = -2 sum_i E_i^T Q_i,
+        dL/dD_c = M^T dL/dC_c,  dL/dM = sum_c dL/dC_c D_c^T.
+    Returns a Factors of gradients.  No reference counterpart (the reference never
+    differentiates the loss)."""
+    M = F.M.astype(dt)
+    gA, gB, gD = [], [], []
+    gM = np.zeros_like(M)
+    for c, cd in enumerate(data):
+        A, B, D = F.A[c].astype(dt), F.B[c].astype(dt), F.D[c].astype(dt)
+        C = M @ D
+        ga, gb, gc = np.zeros_like(A), np.zeros_like(B), np.zeros_like(C)
+        for t, tile in enumerate(cd.tiles):
+            rows = slice(tile.start, tile.start + tile.rows)
+            P = projections[c][t].astype(dt)                          # (rows, w, s)
+            PB = P @ B                                                # (rows, w, s)
+            Q = PB * A[rows][:, None, :]
+            E = cd.X[t].astype(dt) - Q @ C.T                          # (rows, w, N)
+            GQ = -2.0 * (E @ C)                                       # (rows, w, s)
+            ga[rows] += np.einsum("iwr,iwr->ir", GQ, PB)
+            gb += np.einsum("iwp,iwr->pr", P, GQ * A[rows][:, None, :])
+            gc += -2.0 * np.einsum("iwn,iwr->nr", E, Q)
+        gA.append(ga)
+        gB.append(gb)
+        gD.append(M.T @ gc)
+        gM += gc @ D.T
+    return Factors(gA, gB, gD, gM)
