@@ -327,6 +327,8 @@ class SweepEngine:
         self.core_cur = 0
         self.hist_w = 3 + 3 * self.K
         self.hist = z(n_iter_max_hist, self.hist_w, dt=torch.float64)
+        self.hist_stage = z(1, self.hist_w, dt=torch.float64)
+        self._graphs = {}
         self.seg = torch.tensor(np.concatenate([[0], np.cumsum([c.n for c in self.chroms])]),
                                 dtype=torch.int64, device=dev)
         self.iteration = 0
@@ -600,27 +602,56 @@ class SweepEngine:
             out[n0] = out.get(n0, 0.0) + e0.elapsed_time(e1)
         return out
 
-    def sweep(self, n_iter_parafac: Optional[int] = None, stream=None):
-        """One ALS iteration; the errors land in self.hist[self.iteration] (device)."""
-        st = ops.stream_handle(stream)
-        mk = lambda name: self._mark(name, stream)
+    def sweep(self, n_iter_parafac: Optional[int] = None, stream=None, graph: bool = False):
+        """One ALS iteration; the errors land in self.hist[self.iteration] (device).
+
+        graph=True replays the sweep from a hipGraph captured on first use (one graph per
+        (n_iter_parafac, core-norm buffer) pair; the errors go to a fixed staging row that a
+        copy after the replay moves to hist[iteration]).  Same launches, same results."""
         it = self.iteration
         if n_iter_parafac is None:
             if it % 10 == 0 and it > 0 and self.n_iter_parafac < 10:       # :636-637
                 self.n_iter_parafac += 1
             n_iter_parafac = self.n_iter_parafac
+        if it >= self.hist.shape[0]:
+            raise ValueError("history holds %d iterations (n_iter_max_hist)" % self.hist.shape[0])
+        row = self.hist.data_ptr() + 8 * self.hist_w * it
+        if graph:
+            if self.timing is not None:
+                raise ValueError("per-phase timing markers cannot be recorded inside a graph")
+            key = (n_iter_parafac, self.core_cur)
+            g = self._graphs.get(key)
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                torch.cuda.synchronize()
+                with torch.cuda.graph(g):
+                    self._sweep_body(n_iter_parafac, self.hist_stage.data_ptr(), None)
+                self._graphs[key] = g
+            if stream is not None:
+                with torch.cuda.stream(stream):
+                    g.replay()
+            else:
+                g.replay()
+            L.call("fh_copy2d", self.hist_stage.data_ptr(), F64, self.hist_w, 1, row, F64, self.hist_w, 1,
+                   self.hist_w, 1.0, ops.stream_handle(stream))
+        else:
+            self._sweep_body(n_iter_parafac, row, stream)
+        self.core_cur ^= 1
+        self.iteration += 1
+
+    def _sweep_body(self, n_iter_parafac: int, hist_row: int, stream=None):
+        st = ops.stream_handle(stream)
+        mk = lambda name: self._mark(name, stream)
         self.passes(stream)
         # errors with the previous core norm (:667-668)
         prev = self.core[self.core_cur]
         L.call("fh_sweep_errors", self.tnorm.data_ptr(), prev.data_ptr(), self.xU.data_ptr(),
-               self.pbm.nuc.data_ptr(), self.K, self.hist.data_ptr() + 8 * self.hist_w * it, st)
+               self.pbm.nuc.data_ptr(), self.K, hist_row, st)
         # CP-ALS on every T_c, then the new core norm (:674-706)
         mk("cpals")
         self._cpals(n_iter_parafac, stream)
-        self.core_cur ^= 1
-        self._core_norms(self.core_cur, stream)
+        self._core_norms(self.core_cur ^ 1, stream)
         mk("end")
-        self.iteration += 1
 
     def passes(self, stream=None):
         """update_meta_embedding_interactions (parafac2_intergrative.py:303-540): new P_i, x_U,

@@ -18,7 +18,7 @@ Backward (recomputes C^T and Q from the saved factors; gradient = MTTKRP minus f
     F_c   = X_c C_c - Q_c (C_c^T C_c)          X read once (fp32 GEMM), Gram in fp64
     H_i   = P_i^T F_i                          per-bin batched GEMM
     dA, dB from H                              fh_loss_bwd (fp64 accumulation, fixed order)
-    dC_c  = 2 C_c (Q_c^T Q_c) - 2 X_c^T Q_c    X read a second time (k-split), Gram in fp64
+    dC_c  = 2 C_c (Q_c^T Q_c) - 2 X_c^T Q_c    X read a second time; both k-split, partials summed in fp64
     dD_c  = M^T dC_c,  dM = sum_c dC_c D_c^T
 
 With P_i^T P_i = I these are the familiar CP gradients, e.g. dA = 2 (A (B^T B o C^T C) - MTTKRP);
@@ -75,7 +75,7 @@ class LossPlan:
         self.H = z(h)
         nb = eng.B_all.numel()           # sum s_c sp_c; B_all layout for every s x sp block
         self.Gc = z(nb)
-        self.Gq = z(nb, dt=torch.float64)
+        self.Gq = z(nb)
         self.GC = z(Np, Dld)             # dL/dC_c at columns [doff, doff + s)
         self.gA = z(eng.A_all.numel())
         self.gB = z(nb)
@@ -160,19 +160,19 @@ class LossPlan:
         d = [make_desc(Xp + 4 * cl.xrow * Np, f4(self.Q, cl.yoff), f4(self.Zpart, cl.doff), N, cl.s, cl.W, 1, Np,
                        cl.sp, 1, Dld, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=ksz, sCk=Np * Dld) for cl in ch]
         self.g_zx = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
-        # (10) Gq = Q^T Q (fp64), k-split partials
-        t64 = sum(-(-cl.s // 64) ** 2 for cl in ch)
-        ksq = max(1, min(64, -(-1024 // t64), min(cl.W for cl in ch) // 512 or 1))
+        # (10) Gq = Q^T Q: k-split fp32 partials over the positions (about 1024 workgroups, chunks of
+        # at least 256 positions), summed in fp64 in fixed order
+        tq = sum(-(-cl.s // ops.SHAPES[k][0]) * -(-cl.s // ops.SHAPES[k][1]) for k, cl in zip(shp, ch))
+        ksq = max(1, min(256, -(-1024 // tq), min(cl.W for cl in ch) // 256 or 1))
         self.ksq = ksq
-        self.Gqpart = z(ksq * nb, dt=torch.float64)
-        d = [make_desc(f4(self.Q, cl.yoff), f4(self.Q, cl.yoff), self.Gqpart.data_ptr() + 8 * cl.boff, cl.s, cl.s,
-                       cl.W, 1, cl.sp, cl.sp, 1, cl.sp, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=ksq, sCk=nb)
-             for cl in ch]
-        self.g_gq = ops.Gemm64Group(d, F32, F32, F64, dev)
+        self.Gqpart = z(ksq * nb)
+        d = [make_desc(f4(self.Q, cl.yoff), f4(self.Q, cl.yoff), f4(self.Gqpart, cl.boff), cl.s, cl.s, cl.W,
+                       1, cl.sp, cl.sp, 1, cl.sp, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=ksq, sCk=nb) for cl in ch]
+        self.g_gq = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
         # (11) dC = 2 C Gq - 2 X^T Q  (in place over the reduced X^T Q)
-        d = [make_desc(f4(self.Ct, cl.doff * Np), self.Gq.data_ptr() + 8 * cl.boff, f4(self.GC, cl.doff), N, cl.s,
-                       cl.s, 1, Np, cl.sp, 1, Dld, 1, alpha=2.0, beta=-2.0) for cl in ch]
-        self.g_dc = ops.Gemm64Group(d, F32, F64, F32, dev)
+        d = [make_desc(f4(self.Ct, cl.doff * Np), f4(self.Gq, cl.boff), f4(self.GC, cl.doff), N, cl.s, cl.s,
+                       1, Np, cl.sp, 1, Dld, 1, alpha=2.0, beta=-2.0) for cl in ch]
+        self.g_dc = MultiGemm(d, F32, lambda q: shape_for_n(q.N), dev)
         # (12) dD = M^T dC (all chromosomes: R x Dld), (13) dM = dC D^T
         self.g_dd = ops.GemmGroup([make_desc(self.M.data_ptr(), self.GC.data_ptr(), self.gD.data_ptr(), R, Dld, N,
                                              1, R, Dld, 1, Dld, 1)], F32, shape_for_n(Dld), dev)
@@ -218,7 +218,7 @@ class LossPlan:
         Np, Dld = self.eng.Npad, self.eng.Dld
         L.call("fh_batch_sum", self.Zpart.data_ptr(), F32, self.ksz, Np * Dld, Np * Dld, self.GC.data_ptr(), st)
         self.g_gq.run(stream)
-        L.call("fh_batch_sum", self.Gqpart.data_ptr(), F64, self.ksq, nb, nb, self.Gq.data_ptr(), st)
+        L.call("fh_batch_sum", self.Gqpart.data_ptr(), F32, self.ksq, nb, nb, self.Gq.data_ptr(), st)
         self.g_dc.run(stream)
         self.g_dd.run(stream)
         self.g_dm.run(stream)

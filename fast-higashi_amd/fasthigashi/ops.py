@@ -283,6 +283,7 @@ class PolarBatch:
         self.scale = torch.zeros(nmat, dtype=torch.float64, device=device)
         self.res = torch.zeros(n_iter * nmat, dtype=torch.float64, device=device)
         self.nuc = torch.zeros(nmat, dtype=torch.float64, device=device)
+        self.iters = torch.zeros(nmat, dtype=torch.int32, device=device)
         self.offs = []
         groups = []
         off = 0
@@ -336,8 +337,6 @@ class PolarBatch:
         """One workgroup per matrix; G^{-1/2} ends in buffer 0.  method 'newton': scaled Newton
         with in-register Gauss-Jordan inverses (s <= 160, Newton-Schulz above); 'ns': coupled
         Newton-Schulz for every size."""
-        if not hasattr(self, "iters"):
-            self.iters = torch.zeros(self.nmat, dtype=torch.int32, device=self.buf.device)
         if method == "newton":
             L.call("fh_polar_newton", self.table.data_ptr(), len(self.groups), self.mat_begin.data_ptr(),
                    self.nmat, self.newton_iter, self.newton_tol, self.nuc.data_ptr(), self.iters.data_ptr(),
