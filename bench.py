@@ -60,7 +60,7 @@ def launch_work(eng, phase):
 
 
 def kernel_name(phase, head):
-    tag = phase.split(":")[1]
+    tag = phase.split(":")[1].split("#")[0]
     if tag.startswith("s"):
         bm, bn = tag[1:].split("x")
         return "gemm_stream_kernel, bf16 operands, %sx%s tiles, global->LDS ring" % (bm, bn)
@@ -76,7 +76,7 @@ def traffic_from_profiles(kernel_tag):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch[%s]:" % kernel_tag):
-            return float(line.split(":", 1)[1])
+            return float(line.split("]:", 1)[1])
     return None
 
 
@@ -137,10 +137,17 @@ def build(args):
     return wl, coos, tiles, sizes, n_i, (A, B, D, M), engines
 
 
-def time_path(eng, steps, warmup):
+def time_path(eng, steps, warmup, autotune=True):
     import torch
-    for _ in range(warmup):
+    for k in range(max(warmup, 1 if autotune else 0)):
         eng.sweep()
+        if k == 0 and autotune:
+            t0 = time.perf_counter()
+            rep = eng.autotune()
+            torch.cuda.synchronize()
+            log("%s autotune %.1fs: %s" % (eng.dtype, time.perf_counter() - t0, "; ".join(
+                "%s %s" % (fam, ",".join(str(g.get("chosen", "merged" if g.get("merged") else "kept"))
+                                         for g in groups)) for fam, groups in rep.items())))
     torch.cuda.synchronize()
     eng.timing = []
     t0 = time.perf_counter()
@@ -200,6 +207,9 @@ def main():
     ap.add_argument("--cells", type=int, default=0, help="override the cell count (0: the workload's)")
     ap.add_argument("--paths", default="fp32,bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the rule-picked tile shapes (default: time every shape after the first "
+                         "warm-up sweep and keep the fastest; results are the same bit for bit)")
     ap.add_argument("--cpu-cells", type=int, default=5000)
     ap.add_argument("--cpu-chrom", type=int, default=-2, help="-2: the median-size chromosome")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -229,7 +239,7 @@ def main():
         del src, dst
     res = {}
     for dt in args.paths:
-        el, phases = time_path(engines[dt], args.steps, args.warmup)
+        el, phases = time_path(engines[dt], args.steps, args.warmup, not args.no_autotune)
         thr = wl.n_cells * len(coos) * args.steps / el
         hist = engines[dt].history()
         res[dt] = dict(elapsed=el, thr=thr, phases=phases, fit=float(hist["rec_error"][-1]))

@@ -1,0 +1,170 @@
+"""Tile-shape autotuner for the grouped GEMM launches of a SweepEngine.
+
+Every grouped launch of the sweep that reads a large operand (Y = X C, T_hat = P^T X,
+T_tilde = T_hat M, Z = T_hat^T L^T, C^T, temp, SVD_term and the CP-ALS products) is one
+descriptor table launched with one tile shape.  The engine picks shapes by rule (tile height
+covering s, one tile column over s, ...); `tune_engine` instead times every tile shape the
+kernel family has on the engine's own data and keeps the fastest, group by group:
+
+* streaming bf16 GEMM: (bm, bn) in (64 | 96 | 128 | 160, 128) and (128, 256);
+* tiled fp32 / bf16 MFMA GEMM: the seven shapes 128x128, 128x64, 64x64, 128x160, 64x128,
+  256x128, 128x256.
+
+The k order inside a tile does not depend on the tile shape, so every choice gives bitwise the
+same results; only the time changes.  Choices can be kept in a JSON file keyed by the device
+and the group's problem sizes, so a later engine of the same shapes skips the timing.
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Dict, Optional
+
+import torch
+
+from . import ops
+
+STREAM_SHAPES = ((64, 128), (96, 128), (128, 128), (160, 128), (128, 256))
+TILED_SHAPES = tuple(range(7))
+
+# engine attributes holding grouped launches, by family name
+FAMILIES = (("y", "g_y"), ("that", "g_that"), ("ttilde", "g_ttil"), ("ct", "g_ct"), ("temp", "g_temp"),
+            ("z", "g_z"), ("svd", "g_svd"), ("cp_w1", "cpals.g_w1"), ("cp_md", "cpals.g_md"))
+
+
+def _get(obj, path):
+    for p in path.split("."):
+        obj = getattr(obj, p)
+    return obj
+
+
+def _set(obj, path, value):
+    parts = path.split(".")
+    for p in parts[:-1]:
+        obj = getattr(obj, p)
+    setattr(obj, parts[-1], value)
+
+
+def _time_ms(run, reps: int) -> float:
+    run()                                      # warm (code objects, caches)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in evs:
+        e0.record()
+        run()
+        e1.record()
+    torch.cuda.synchronize()
+    t = sorted(e0.elapsed_time(e1) for e0, e1 in evs)
+    return t[len(t) // 2]
+
+
+def _group_key(family: str, g) -> str:
+    dev = torch.cuda.get_device_name()
+    kind = "stream" if isinstance(g, ops.StreamGemmGroup) else "tiled%d" % g.dtype
+    sizes = sorted((d.M, d.N, d.K, d.batch) for d in g.descs)
+    return "%s|%s|%s|%s" % (dev, family, kind, sizes)
+
+
+def _candidates(g, device):
+    if isinstance(g, ops.StreamGemmGroup):
+        return [((bm, bn), (lambda bm=bm, bn=bn: ops.StreamGemmGroup(g.descs, bm=bm, bn=bn, grid=g.grid,
+                                                                      device=device)))
+                for bm, bn in STREAM_SHAPES], (g.bm, g.bn)
+    if isinstance(g, ops.GemmGroup):
+        return [(k, (lambda k=k: ops.GemmGroup(g.descs, g.dtype, k, device))) for k in TILED_SHAPES], g.shape
+    return None, None
+
+
+def _shape_of(g):
+    return (g.bm, g.bn) if isinstance(g, ops.StreamGemmGroup) else g.shape
+
+
+def _merge(fam, groups, device, reps, cache, report, verbose):
+    """Groups of one family that ended on the same shape (and layout) become one launch when
+    that is faster than launching them one after the other."""
+    by = {}
+    for g in groups:
+        if isinstance(g, (ops.StreamGemmGroup, ops.GemmGroup)):
+            k = (type(g).__name__, getattr(g, "dtype", None), getattr(g, "ak", None), getattr(g, "bk", None),
+                 _shape_of(g))
+            by.setdefault(k, []).append(g)
+    out = [g for g in groups if not isinstance(g, (ops.StreamGemmGroup, ops.GemmGroup))]
+    for k, gs in by.items():
+        if len(gs) == 1:
+            out.extend(gs)
+            continue
+        descs = [d for g in gs for d in g.descs]
+        shape = _shape_of(gs[0])
+        make = ((lambda: ops.StreamGemmGroup(descs, bm=shape[0], bn=shape[1], grid=gs[0].grid, device=device))
+                if isinstance(gs[0], ops.StreamGemmGroup) else
+                (lambda: ops.GemmGroup(descs, gs[0].dtype, shape, device)))
+        key = "%s|merge|%s" % (_group_key(fam, gs[0]), sorted((d.M, d.N, d.K, d.batch) for d in descs))
+        if key in cache:
+            merged = cache[key]
+        else:
+            m = make()
+            t_sep = _time_ms(lambda: [g.run() for g in gs], reps)
+            t_one = _time_ms(m.run, reps)
+            merged = t_one < 0.98 * t_sep
+            cache[key] = merged
+            if verbose:
+                print("[autotune] %s: %d groups on %s, one launch %.3f ms vs %.3f ms -> %s"
+                      % (fam, len(gs), shape, t_one, t_sep, "merged" if merged else "kept"), flush=True)
+        report.setdefault(fam + ":merge", []).append(dict(shape=shape, groups=len(gs), merged=merged))
+        out.extend([make()] if merged else gs)
+    return out
+
+
+def tune_engine(eng, reps: int = 5, families=None, cache_path: Optional[str] = None, verbose: bool = False) -> Dict:
+    """Time every tile shape of each grouped launch of `eng` and keep the fastest.
+
+    Run after at least one sweep (the buffers then hold real data).  Returns
+    {family: [{"group": i, "was": shape, "chosen": shape, "ms": {shape: ms}}]}."""
+    cache = {}
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            cache = json.load(f)
+    report = {}
+    for fam, attr in FAMILIES:
+        if families is not None and fam not in families:
+            continue
+        try:
+            multi = _get(eng, attr)
+        except AttributeError:
+            continue
+        single = not hasattr(multi, "groups")
+        groups = [multi] if single else list(multi.groups)
+        new = []
+        for i, g in enumerate(groups):
+            cands, cur = _candidates(g, eng.device)
+            if cands is None:
+                new.append(g)
+                continue
+            key = _group_key(fam, g)
+            ms = {}
+            if key in cache:
+                chosen = cache[key]
+                chosen = tuple(chosen) if isinstance(chosen, list) else chosen
+            else:
+                best = None
+                for shape, make in cands:
+                    cand = g if shape == cur else make()
+                    ms[shape] = _time_ms(cand.run, reps)
+                    if best is None or ms[shape] < ms[best]:
+                        best = shape
+                # keep the rule's shape unless another is at least 2% faster
+                chosen = cur if ms[best] > 0.98 * ms[cur] else best
+                cache[key] = list(chosen) if isinstance(chosen, tuple) else chosen
+            pick = g if chosen == cur else dict(cands)[chosen]()
+            new.append(pick)
+            report.setdefault(fam, []).append(dict(group=i, was=cur, chosen=chosen, ms=ms))
+            if verbose:
+                print("[autotune] %s group %d: %s -> %s %s" % (fam, i, cur, chosen,
+                      ", ".join("%s %.3f" % kv for kv in ms.items())), flush=True)
+        if single:
+            _set(eng, attr, new[0])
+        else:
+            multi.groups = _merge(fam, new, eng.device, reps, cache, report, verbose)
+    if cache_path:
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return report

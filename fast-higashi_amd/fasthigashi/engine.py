@@ -570,18 +570,32 @@ class SweepEngine:
         """phase-name suffix of one grouped launch: s<bm>x<bn> (streaming) or t<shape> (tiled)."""
         return ("s%dx%d" % (g.bm, g.bn)) if isinstance(g, ops.StreamGemmGroup) else ("t%d" % g.shape)
 
+    def _tags(self, prefix, multi):
+        """phase names of a family's launches: prefix:tag, with #2, #3 .. for repeated tags"""
+        out, seen = [], {}
+        for g in multi.groups:
+            t = "%s:%s" % (prefix, self.group_tag(g))
+            seen[t] = seen.get(t, 0) + 1
+            out.append(t if seen[t] == 1 else "%s#%d" % (t, seen[t]))
+        return out
+
     def launch_groups(self) -> dict:
         """phase name -> grouped launch, for the launches timed one by one (Y, T_hat, T_tilde)."""
         out = {}
         for prefix, multi in (("y", self.g_y), ("that", self.g_that), ("ttilde", self.g_ttil)):
-            for g in multi.groups:
-                out["%s:%s" % (prefix, self.group_tag(g))] = g
+            out.update(zip(self._tags(prefix, multi), multi.groups))
         return out
+
+    def autotune(self, reps: int = 5, cache_path: Optional[str] = None, verbose: bool = False):
+        """Time every tile shape of each grouped launch and keep the fastest (see autotune.py).
+        Results are unchanged bit for bit; call after at least one sweep."""
+        from .autotune import tune_engine
+        return tune_engine(self, reps=reps, cache_path=cache_path, verbose=verbose)
 
     def _run_marked(self, prefix, multi, stream):
         # one marker per launch, so each launch is timed on its own
-        for g in multi.groups:
-            self._mark("%s:%s" % (prefix, self.group_tag(g)), stream)
+        for tag, g in zip(self._tags(prefix, multi), multi.groups):
+            self._mark(tag, stream)
             g.run(stream)
 
     def _mark(self, name, stream):

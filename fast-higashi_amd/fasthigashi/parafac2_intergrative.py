@@ -38,6 +38,10 @@ class Fast_Higashi_core:
         self.engine: Optional[SweepEngine] = None
         self.svd_random_state = None
         self.path_dtype = "fp32"
+        # after the first sweep, time every tile shape of the grouped launches and keep the
+        # fastest (results unchanged bit for bit); optional JSON cache of the choices
+        self.autotune = True
+        self.autotune_cache: Optional[str] = None
 
     def to(self, device):
         self.device = torch.device(device)
@@ -209,6 +213,8 @@ class Fast_Higashi_core:
         for iteration in range(n_iter_max):
             start_time = time.time()
             eng.sweep()
+            if iteration == 0 and self.autotune:
+                eng.autotune(cache_path=self.autotune_cache)
             h = eng.hist[iteration].cpu().numpy() if (tol > 0 or verbose) else None
             if h is None:
                 continue

@@ -86,3 +86,40 @@ def test_graph_sweep_matches_eager(cuda, tiny, dtype):
     for a, b in zip(eager.get_factors(), graph.get_factors()):
         for x, y in zip(a if isinstance(a, list) else [a], b if isinstance(b, list) else [b]):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "bf16"])
+def test_autotuned_engine_matches_untuned(cuda, tiny, dtype, tmp_path):
+    """The tuner times every tile shape of each grouped launch and keeps the fastest; the k order
+    inside a tile does not depend on the shape, so the tuned engine's sweeps are bit-identical.
+    A second engine of the same shapes takes its choices from the cache file."""
+    from fasthigashi.autotune import tune_engine
+    wl, data, F0, sizes = tiny
+    ref = _engine(data, F0, sizes, wl.rank, dtype)
+    tuned = _engine(data, F0, sizes, wl.rank, dtype)
+    ref.sweep()
+    tuned.sweep()
+    cache = str(tmp_path / "tune.json")
+    rep = tune_engine(tuned, reps=3, cache_path=cache)
+    assert "y" in rep and "that" in rep and all(len(v) > 0 for v in rep.values())
+    for fam in ("y", "that", "ttilde"):
+        assert all(isinstance(g, type(tuned.g_y.groups[0])) for g in getattr(tuned, {"y": "g_y", "that": "g_that",
+                                                                                      "ttilde": "g_ttil"}[fam]).groups)
+    for k in range(1, 4):
+        ref.sweep()
+        tuned.sweep()
+        torch.cuda.synchronize()
+        assert np.array_equal(ref.hist[k].cpu().numpy(), tuned.hist[k].cpu().numpy()), k
+    again = _engine(data, F0, sizes, wl.rank, dtype)
+    again.sweep()
+    rep2 = tune_engine(again, reps=3, cache_path=cache)
+    for fam, groups in rep2.items():
+        for g, g1 in zip(groups, rep[fam]):
+            if "ms" in g:
+                assert g["ms"] == {} and g["chosen"] == g1["chosen"], fam
+            else:
+                assert g["merged"] == g1["merged"], fam
+    for k in range(1, 3):
+        again.sweep()
+        torch.cuda.synchronize()
+        assert np.array_equal(ref.hist[k].cpu().numpy(), again.hist[k].cpu().numpy()), k
