@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fast-higashi_amd"))
 
 METRIC = "cell·chrom ALS sweeps/s at 10k cells r256: fused BF16 vs plain FP32, equal fit"
-PEAK = {"hbm": 8.0e12, "bf16": 2.5e15, "fp32": 157.3e12, "fp64": 78.6e12}
+PEAK = {"hbm": 8.0e12, "bf16": 2.5e15, "fp8": 5.0e15, "fp32": 157.3e12, "fp64": 78.6e12}
 
 
 def log(*a):
@@ -44,7 +44,7 @@ def launch_work(eng, phase):
     if g is None:
         return None
     if isinstance(g, ops.StreamGemmGroup):
-        ea = eb = 2
+        ea = eb = 1 if g.fp8 else 2
     else:
         ea = eb = {L.F32: 4, L.BF16: 2, L.FP8: 1}[g.dtype]
     fl = by = 0.0
@@ -54,7 +54,7 @@ def launch_work(eng, phase):
         fl += d.batch * 2.0 * d.M * d.N * d.K
         by += d.batch * float(d.M * d.K * ea + d.K * d.N * eb + d.M * d.N * ec)
         items += d.batch
-    peak = PEAK["bf16" if eb == 2 else "fp32"]
+    peak = PEAK[{1: "fp8", 2: "bf16"}.get(eb, "fp32")]
     bound = "hbm" if fl / by < peak / PEAK["hbm"] else "mfma"
     return bound, by, fl, "%s: %d descriptors, %d products" % (WHAT[phase.split(":")[0]], len(g.descs), items)
 
@@ -63,7 +63,8 @@ def kernel_name(phase, head):
     tag = phase.split(":")[1].split("#")[0]
     if tag.startswith("s"):
         bm, bn = tag[1:].split("x")
-        return "gemm_stream_kernel, bf16 operands, %sx%s tiles, global->LDS ring" % (bm, bn)
+        opnd = "fp8 (e4m3)" if head == "fp8" and not phase.startswith("ttilde") else "bf16"
+        return "gemm_stream_kernel, %s operands, %sx%s tiles, global->LDS ring" % (opnd, bm, bn)
     from fasthigashi import ops
     return "gemm_mfma_kernel, %s operands, %dx%d tiles" % (head, *ops.SHAPES[int(tag[1:])])
 
@@ -122,10 +123,11 @@ def build(args):
     for dt in args.paths:
         t0 = time.time()
         eng = SweepEngine(tiles, n_bins, sizes, wl.n_cells, wl.rank, dtype=dt)
-        if "fp32" in engines:
-            src = engines["fp32"].X
-            L.call("fh_copy2d", src.data_ptr(), L.F32, src.numel(), 1, eng.X.data_ptr(), eng.xdt, src.numel(),
-                   1, src.numel(), 1.0, ops.stream_handle())
+        have = [e for e in engines.values() if e.X is not None]
+        if have:
+            src = have[0].X
+            L.call("fh_copy2d", src.data_ptr(), have[0].xdt, src.numel(), 1, eng.X.data_ptr(), eng.xdt,
+                   src.numel(), 1, src.numel(), 1.0, ops.stream_handle())
             eng.finish_X()
         else:
             build_X(eng, coos, n_i, do_rwr=wl.do_rwr)
@@ -260,13 +262,13 @@ def main():
         roof = {"bound": "hbm", "achieved": nbytes / sec / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
                 "frac": nbytes / sec / PEAK["hbm"], "traffic": traffic_from_profiles(dominant)}
     else:
-        pk = PEAK["bf16" if head == "bf16" else "fp32"]
+        pk = PEAK[head if head in ("bf16", "fp8") else "fp32"]
         roof = {"bound": "mfma", "achieved": nflops / sec / 1e12, "peak": pk / 1e12, "unit": "TFLOP/s",
                 "frac": nflops / sec / pk, "traffic": traffic_from_profiles(dominant)}
     roof.update({"kernel": "%s (%s): %s" % (WHAT[dominant.split(":")[0]], dominant, kname),
                  "launch_ms": phases[dominant],
                  "algorithmic_bytes": nbytes, "algorithmic_flops": nflops, "work": what,
-                 "mfma_frac": nflops / sec / PEAK["bf16" if head == "bf16" else "fp32"]})
+                 "mfma_frac": nflops / sec / PEAK[head if head in ("bf16", "fp8") else "fp32"]})
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()

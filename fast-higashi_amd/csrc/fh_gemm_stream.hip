@@ -32,6 +32,7 @@ typedef __attribute__((address_space(3))) void* lds_ptr_t;
 typedef __attribute__((address_space(1))) uint16_t gu16;
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) u32x4 gu32x4;
+typedef int v4i_t __attribute__((ext_vector_type(4)));
 
 __device__ __attribute__((aligned(16))) uint32_t g_zero16[4];
 
@@ -64,12 +65,16 @@ __device__ __forceinline__ TilePos tile_pos(CDesc& d, int64_t t, int BM, int BN)
 // what the output of one tile needs
 struct StoreRef {
   void* C;           // element (m0, n0) of the output (batch applied)
+  const float* sa;   // fp8: row scales from m0 (nullable)
+  const float* sb;   // fp8: column scales from n0 (nullable)
   int64_t sCm, sCn, rows, cols;
   float alpha;
   int nk;
   bool live, bf16, vec;   // vec: C n-contiguous with rows 16-byte aligned
 };
 
+// EK: elements per k-step (64 bf16 or 128 fp8, one 128-byte chunk-row either way)
+template <int EK>
 __device__ __forceinline__ StoreRef store_ref(CDesc* descs, int n_desc, int& di, int64_t t, int64_t total, int BM,
                                               int BN) {
   StoreRef r{};
@@ -89,7 +94,9 @@ __device__ __forceinline__ StoreRef store_ref(CDesc* descs, int n_desc, int& di,
   r.rows = d.M - p.m0;
   r.cols = d.N - p.n0;
   r.alpha = (float)d.alpha;
-  r.nk = (int)((d.K + 63) / 64);
+  r.nk = (int)((d.K + EK - 1) / EK);
+  r.sa = d.scale_a ? d.scale_a + p.b * d.sSa + p.m0 : nullptr;
+  r.sb = d.scale_b ? d.scale_b + p.b * d.sSb + p.n0 : nullptr;
   r.live = true;
   const int al = r.bf16 ? 7 : 3;   // outputs per 16 bytes, minus one
   r.vec = (((uintptr_t)d.C) & 15) == 0 && d.sCn == 1 && (d.sCm & al) == 0 && (d.sCb & al) == 0;
@@ -121,15 +128,16 @@ struct StreamCfg {
 template <int D>
 struct LoadRef {
   const uint8_t* p[D];
-  uint32_t kstep;    // 128 (bytes of 64 bf16), 0 for a tile past the end
+  uint32_t kstep;    // 128 (bytes of 64 bf16 or 128 fp8), 0 for a tile past the end
   int nk;            // k-steps
-  int nv_last;       // 8-element chunks of the last k-step inside K (8: all)
+  int nv_last;       // 16-byte chunks of the last k-step inside K (8: all)
   bool live;
 };
 
-template <class C>
+template <class C, bool F8>
 __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int& di, int64_t t, int64_t total,
                                                   int wave, int lane) {
+  constexpr int ES = F8 ? 1 : 2, EK = 128 / ES;    // bytes per element, elements per k-step
   LoadRef<C::D> r;
   if (t >= total) {
 #pragma unroll
@@ -143,8 +151,8 @@ __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int&
   di = advance_desc(descs, n_desc, di, t);
   CDesc& d = descs[di];
   const TilePos pos = tile_pos(d, t, C::BM, C::BN);
-  const uint8_t* baseA = (const uint8_t*)((const __bf16*)d.A + pos.b * d.sAb + pos.m0 * d.sAm);
-  const uint8_t* baseB = (const uint8_t*)((const __bf16*)d.B + pos.b * d.sBb + pos.n0 * d.sBn);
+  const uint8_t* baseA = (const uint8_t*)d.A + ES * (pos.b * d.sAb + pos.m0 * d.sAm);
+  const uint8_t* baseB = (const uint8_t*)d.B + ES * (pos.b * d.sBb + pos.n0 * d.sBn);
   const int64_t rows_a = d.M - pos.m0, rows_b = d.N - pos.n0;
 #pragma unroll
   for (int q = 0; q < C::D; ++q) {
@@ -154,11 +162,11 @@ __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int&
     const int row = blk * 8 + (lane >> 3);
     const int c = (lane & 7) ^ (row & 7);
     const int64_t rr = min((int64_t)row, (isA ? rows_a : rows_b) - 1);
-    r.p[q] = (isA ? baseA : baseB) + 2 * rr * (isA ? d.sAm : d.sBn) + 16 * c;
+    r.p[q] = (isA ? baseA : baseB) + ES * rr * (isA ? d.sAm : d.sBn) + 16 * c;
   }
   r.kstep = 128;
-  r.nk = (int)((d.K + 63) / 64);
-  r.nv_last = (int)((d.K - 64 * (int64_t)(r.nk - 1) + 7) / 8);
+  r.nk = (int)((d.K + EK - 1) / EK);
+  r.nv_last = (int)(((d.K - EK * (int64_t)(r.nk - 1)) * ES + 15) / 16);
   r.live = true;
   return r;
 }
@@ -197,12 +205,41 @@ __device__ __forceinline__ void wait_vm_barrier() {
   asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" : : "n"(N) : "memory");
 }
 
-template <class C>
+template <class C, bool F8>
 __device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[C::WM_T][C::WN_T], int wm, int wn,
                                               int lane) {
   constexpr int WM_T = C::WM_T, WN_T = C::WN_T;
   const uint8_t* As = st;
   const uint8_t* Bs = st + C::BM * 128;
+  if constexpr (F8) {
+    // fp8: the lane's two chunks are 32 consecutive k of the 128-wide step (k = 32 (l / 16) ..),
+    // read straight into the 8-register MFMA operands; one MFMA 16x16x128 per block, scale
+    // operands 127 = 2^0 (scales are applied in the epilogue)
+    typedef int v8i __attribute__((ext_vector_type(8)));
+    v8i bf8[WN_T], af8[WM_T];
+#pragma unroll
+    for (int j = 0; j < WN_T; ++j) {
+      const int row = wn * WN_T * 16 + j * 16 + (lane & 15);
+      const int c = 2 * (lane >> 4);
+      const v4i_t lo = *(const v4i_t*)(Bs + row * 128 + ((c ^ (row & 7)) << 4));
+      const v4i_t hi = *(const v4i_t*)(Bs + row * 128 + (((c + 1) ^ (row & 7)) << 4));
+      bf8[j] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i) {
+      const int row = wm * WM_T * 16 + i * 16 + (lane & 15);
+      const int c = 2 * (lane >> 4);
+      const v4i_t lo = *(const v4i_t*)(As + row * 128 + ((c ^ (row & 7)) << 4));
+      const v4i_t hi = *(const v4i_t*)(As + row * 128 + (((c + 1) ^ (row & 7)) << 4));
+      af8[i] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+      for (int j = 0; j < WN_T; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(bf8[j], af8[i], acc[i][j], 0, 0, 0, 127, 0, 127);
+    return;
+  }
   u32x4 bf[WN_T][2];
 #pragma unroll
   for (int j = 0; j < WN_T; ++j) {
@@ -242,7 +279,7 @@ __device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[
 // lane reads back 16 bytes of row l / 8 (chunk l % 8), so one store covers 8 full 128-byte
 // rows of C.  Rows that are not 16-byte aligned, and the ragged n edge, are written element
 // by element from the staged rows (a rolled loop).
-template <int ESZ, class C>
+template <int ESZ, class C, bool F8>
 __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C::WM_T][C::WN_T], int wm, int wn,
                                            int lane, uint8_t* so) {
   constexpr int WM_T = C::WM_T, WN_T = C::WN_T;
@@ -253,6 +290,21 @@ __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C:
   const float alpha = sr.alpha;
   const int64_t sCm = sr.sCm, sCn = sr.sCn, rows = sr.rows, cols = sr.cols;
   const int half = (lane >> 3) & 1;                // which 8 of the block's 16 rows this lane holds
+  // fp8: the lane's row and column scales, loaded once per tile (indices clamped to the valid
+  // rows / columns; the clamped ones belong to outputs that are not stored)
+  float fa[WM_T];
+  floatx4 fb[WN_T];
+  if constexpr (F8) {
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i)
+      fa[i] = sr.sa ? sr.sa[min((int64_t)(wm * WM_T * 16 + i * 16 + (lane & 15)), rows - 1)] * alpha : alpha;
+#pragma unroll
+    for (int j = 0; j < WN_T; ++j) {
+      const int64_t n0 = wn * WN_T * 16 + j * 16 + (lane >> 4) * 4;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) fb[j][r] = sr.sb ? sr.sb[min(n0 + r, cols - 1)] : 1.f;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < WM_T; ++i) {
     const int64_t mb = wm * WM_T * 16 + i * 16;
@@ -265,7 +317,9 @@ __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C:
           if (half == hh) {
 #pragma unroll
             for (int jj = 0; jj < JPG; ++jj) {
-              const floatx4 v = acc[i][g * JPG + jj] * alpha;
+              floatx4 v;
+              if constexpr (F8) v = acc[i][g * JPG + jj] * fa[i] * fb[g * JPG + jj];
+              else v = acc[i][g * JPG + jj] * alpha;
               uint8_t* dst = so + (lane & 7) * kStageRow + (jj * 16 + (lane >> 4) * 4) * ESZ;
               if (ESZ == 2) {
                 u32x2 w;
@@ -305,10 +359,11 @@ __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C:
 // last tile).  The output
 // stores count in vmcnt too; the stage wait stays exact because a wave's loads retire in
 // issue order, so vmcnt <= n leaves at most its n youngest loads outstanding.
-template <class C>
+template <class C, bool F8>
 __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __restrict__ descs_arg, int n_desc,
                                                             int64_t total_tiles) {
   constexpr int WM_T = C::WM_T, WN_T = C::WN_T, STAGES = C::STAGES;
+  constexpr int EK = F8 ? 128 : 64;
   CDesc* descs = (CDesc*)descs_arg;
   __shared__ __attribute__((aligned(1024))) uint8_t ring[STAGES * C::STAGE_BYTES];
   __shared__ __attribute__((aligned(16))) uint8_t SO[C::NW][8 * kStageRow];
@@ -321,12 +376,12 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
   // load cursor
   int64_t lt = blockIdx.x;
   int dl = 0;
-  LoadRef<C::D> lr = load_ref<C>(descs, n_desc, dl, lt, total_tiles, wave, lane);
+  LoadRef<C::D> lr = load_ref<C, F8>(descs, n_desc, dl, lt, total_tiles, wave, lane);
   int lk = 0;
   // compute cursor
   int64_t ct = blockIdx.x;
   int dc = 0;
-  StoreRef sr = store_ref(descs, n_desc, dc, ct, total_tiles, C::BM, C::BN);
+  StoreRef sr = store_ref<EK>(descs, n_desc, dc, ct, total_tiles, C::BM, C::BN);
   int ck = 0;
 
   floatx4 acc[WM_T][WN_T];
@@ -340,7 +395,7 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
     if (++lk >= lr.nk) {
       lk = 0;
       lt += stride;
-      lr = load_ref<C>(descs, n_desc, dl, lt, total_tiles, wave, lane);
+      lr = load_ref<C, F8>(descs, n_desc, dl, lt, total_tiles, wave, lane);
     }
   };
 #pragma unroll
@@ -348,17 +403,17 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
   for (int k = 0; sr.live; ++k) {
     wait_vm_barrier<C::D * (STAGES - 2)>();
     issue((k + STAGES - 1) % STAGES);
-    compute_stage<C>(ring + (k % STAGES) * C::STAGE_BYTES, acc, wm, wn, lane);
+    compute_stage<C, F8>(ring + (k % STAGES) * C::STAGE_BYTES, acc, wm, wn, lane);
     if (++ck >= sr.nk) {
-      if (sr.bf16) store_tile<2, C>(sr, acc, wm, wn, lane, SO[wave]);
-      else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);
+      if (sr.bf16) store_tile<2, C, F8>(sr, acc, wm, wn, lane, SO[wave]);
+      else store_tile<4, C, F8>(sr, acc, wm, wn, lane, SO[wave]);
 #pragma unroll
       for (int i = 0; i < WM_T; ++i)
 #pragma unroll
         for (int j = 0; j < WN_T; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
       ck = 0;
       ct += stride;
-      sr = store_ref(descs, n_desc, dc, ct, total_tiles, C::BM, C::BN);
+      sr = store_ref<EK>(descs, n_desc, dc, ct, total_tiles, C::BM, C::BN);
     }
   }
   asm volatile("s_waitcnt vmcnt(0)" : : : "memory");   // no load may still target LDS at exit
@@ -370,16 +425,17 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
 // (bm, bn): (64 | 96 | 128 | 160, 128) with 4 waves and 4 stages, or (128, 256) with 8 waves
 // and 3 stages.  grid: persistent workgroups (0 = one per CU).
 namespace {
-template <class C>
+template <class C, bool F8>
 int launch_stream(const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL((gemm_stream_kernel<C>), dim3(grid), dim3(C::NT), 0, stream, descs_dev, n_desc, total_tiles);
+  hipLaunchKernelGGL((gemm_stream_kernel<C, F8>), dim3(grid), dim3(C::NT), 0, stream, descs_dev, n_desc,
+                     total_tiles);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }
-}  // namespace
 
-extern "C" int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
-                                     int grid, hipStream_t stream) {
+template <bool F8>
+int stream_dispatch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles, int grid,
+                    hipStream_t stream) {
   if (total_tiles <= 0) return FH_OK;
   FH_REQUIRE(descs_dev != nullptr && n_desc > 0, "gemm_stream: descriptor table required");
   FH_REQUIRE(total_tiles < ((int64_t)1 << 31), "gemm_stream: %lld tiles exceed the 32-bit tile index",
@@ -393,15 +449,26 @@ extern "C" int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev
   if ((int64_t)grid > total_tiles) grid = (int)total_tiles;
   if (bn == 128) {
     switch (bm) {
-      case 64: return launch_stream<StreamCfg<2, 4, 2, 2, 4>>(descs_dev, n_desc, total_tiles, grid, stream);
-      case 96: return launch_stream<StreamCfg<3, 4, 2, 2, 4>>(descs_dev, n_desc, total_tiles, grid, stream);
-      case 128: return launch_stream<StreamCfg<4, 4, 2, 2, 4>>(descs_dev, n_desc, total_tiles, grid, stream);
-      case 160: return launch_stream<StreamCfg<5, 4, 2, 2, 4>>(descs_dev, n_desc, total_tiles, grid, stream);
+      case 64: return launch_stream<StreamCfg<2, 4, 2, 2, 4>, F8>(descs_dev, n_desc, total_tiles, grid, stream);
+      case 96: return launch_stream<StreamCfg<3, 4, 2, 2, 4>, F8>(descs_dev, n_desc, total_tiles, grid, stream);
+      case 128: return launch_stream<StreamCfg<4, 4, 2, 2, 4>, F8>(descs_dev, n_desc, total_tiles, grid, stream);
+      case 160: return launch_stream<StreamCfg<5, 4, 2, 2, 4>, F8>(descs_dev, n_desc, total_tiles, grid, stream);
       default: break;
     }
   } else if (bn == 256 && bm == 128) {
-    return launch_stream<StreamCfg<4, 4, 2, 4, 3>>(descs_dev, n_desc, total_tiles, grid, stream);
+    return launch_stream<StreamCfg<4, 4, 2, 4, 3>, F8>(descs_dev, n_desc, total_tiles, grid, stream);
   }
   fh_set_error("gemm_stream: unsupported tile %d x %d", bm, bn);
   return FH_ERR_ARG;
+}
+}  // namespace
+
+extern "C" int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
+                                     int grid, hipStream_t stream) {
+  return stream_dispatch<false>(bm, bn, descs_dev, n_desc, total_tiles, grid, stream);
+}
+
+extern "C" int fh_gemm_stream8_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
+                                      int grid, hipStream_t stream) {
+  return stream_dispatch<true>(bm, bn, descs_dev, n_desc, total_tiles, grid, stream);
 }

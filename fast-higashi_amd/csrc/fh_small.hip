@@ -199,14 +199,15 @@ __global__ __launch_bounds__(NT) void transpose_grouped_kernel(const FhTransDesc
   local -= b * per;
   const int64_t r0 = (local / d.tiles_c) * 64, c0 = (local % d.tiles_c) * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const char* src = (const char*)d.src + b * d.sbs * (d.sdt == FH_BF16 ? 2 : 4);
-  char* dst = (char*)d.dst + b * d.sbd * (d.ddt == FH_BF16 ? 2 : 4);
+  const int es = d.sdt == FH_BF16 ? 2 : (d.sdt == FH_FP8 ? 1 : 4), ed = d.ddt == FH_BF16 ? 2 : (d.ddt == FH_FP8 ? 1 : 4);
+  const char* src = (const char*)d.src + b * d.sbs * es;
+  char* dst = (char*)d.dst + b * d.sbd * ed;
   for (int i = ty; i < 64; i += NT / 64) {
     const int64_t r = r0 + i, c = c0 + tx;
     float v = 0.f;
     if (r < d.rows && c < d.cols) {
       const int64_t o = r * d.lds + c;
-      v = d.sdt == FH_BF16 ? (float)((const __bf16*)src)[o] : ((const float*)src)[o];
+      v = es == 2 ? (float)((const __bf16*)src)[o] : (es == 1 ? (float)((const uint8_t*)src)[o] : ((const float*)src)[o]);
     }
     tile[i][tx] = v;
   }
@@ -216,7 +217,8 @@ __global__ __launch_bounds__(NT) void transpose_grouped_kernel(const FhTransDesc
     if (r < d.rows && c < d.cols) {
       const int64_t o = c * d.ldd + r;
       const float v = tile[tx][i];
-      if (d.ddt == FH_BF16) ((uint16_t*)dst)[o] = fh_f32_to_bf16_bits(v);
+      if (ed == 2) ((uint16_t*)dst)[o] = fh_f32_to_bf16_bits(v);
+      else if (ed == 1) ((uint8_t*)dst)[o] = (uint8_t)v;     // FP8 bytes (FP8 source only)
       else ((float*)dst)[o] = v;
     }
   }

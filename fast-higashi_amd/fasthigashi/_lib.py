@@ -29,6 +29,7 @@ class GemmDesc(Structure):
         ("alpha", c_double), ("beta", c_double), ("diag", c_double),
         ("scale_a", c_void_p), ("scale_b", c_void_p), ("skip", c_void_p),
         ("tile_begin", c_int64), ("tiles_m", c_int32), ("tiles_n", c_int32),
+        ("sSa", c_int64), ("sSb", c_int64),
     ]
 
 
@@ -69,6 +70,13 @@ class CpChrom(Structure):
                 ("state", c_void_p), ("flag", c_void_p), ("core", c_void_p),
                 ("n", c_int64), ("s", c_int64), ("sp", c_int64), ("R", c_int64), ("Dld", c_int64),
                 ("mdsplit", c_int64)]
+
+
+class QuantRows(Structure):
+    _fields_ = [("src", c_void_p), ("scale", c_void_p), ("dst", c_void_p), ("colscale", c_void_p),
+                ("rows", c_int64), ("cols", c_int64), ("lds", c_int64), ("ldd", c_int64),
+                ("cs_rows", c_int64), ("cs_stride", c_int64),
+                ("sdt", c_int32), ("pad", c_int32), ("row_begin", c_int64)]
 
 
 class LossChrom(Structure):
@@ -133,6 +141,14 @@ _SIGS = {
                             c_void_p]),
     "fh_colsum_add": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64,
                               c_void_p]),
+    "fh_gemm_stream8_launch": (c_int, [c_int, c_int, c_void_p, c_int, c_int64, c_int, c_void_p]),
+    "fh_amax_rows": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "fh_amax_col_chunks": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                                   c_void_p]),
+    "fh_quant_col_chunks": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                                    c_void_p, c_int64, c_void_p, c_void_p]),
+    "fh_quant_rows_prepare": (c_int64, [c_void_p, c_int]),
+    "fh_quant_rows": (c_int, [c_void_p, c_int, c_int64, c_void_p]),
     "fh_loss_fwd": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
     "fh_loss_prepare": (c_int64, [c_void_p, c_int]),
     "fh_loss_bwd": (c_int, [c_void_p, c_int, c_int64, c_float, c_void_p]),

@@ -6,7 +6,7 @@ descriptor table launched with one tile shape.  The engine picks shapes by rule 
 covering s, one tile column over s, ...); `tune_engine` instead times every tile shape the
 kernel family has on the engine's own data and keeps the fastest, group by group:
 
-* streaming bf16 GEMM: (bm, bn) in (64 | 96 | 128 | 160, 128) and (128, 256);
+* streaming bf16 / fp8 GEMM: (bm, bn) in (64 | 96 | 128 | 160, 128) and (128, 256);
 * tiled fp32 / bf16 MFMA GEMM: the seven shapes 128x128, 128x64, 64x64, 128x160, 64x128,
   256x128, 128x256.
 
@@ -59,7 +59,7 @@ def _time_ms(run, reps: int) -> float:
 
 def _group_key(family: str, g) -> str:
     dev = torch.cuda.get_device_name()
-    kind = "stream" if isinstance(g, ops.StreamGemmGroup) else "tiled%d" % g.dtype
+    kind = ("stream8" if g.fp8 else "stream") if isinstance(g, ops.StreamGemmGroup) else "tiled%d" % g.dtype
     sizes = sorted((d.M, d.N, d.K, d.batch) for d in g.descs)
     return "%s|%s|%s|%s" % (dev, family, kind, sizes)
 
@@ -67,7 +67,7 @@ def _group_key(family: str, g) -> str:
 def _candidates(g, device):
     if isinstance(g, ops.StreamGemmGroup):
         return [((bm, bn), (lambda bm=bm, bn=bn: ops.StreamGemmGroup(g.descs, bm=bm, bn=bn, grid=g.grid,
-                                                                      device=device)))
+                                                                      device=device, fp8=g.fp8)))
                 for bm, bn in STREAM_SHAPES], (g.bm, g.bn)
     if isinstance(g, ops.GemmGroup):
         return [(k, (lambda k=k: ops.GemmGroup(g.descs, g.dtype, k, device))) for k in TILED_SHAPES], g.shape
@@ -85,7 +85,7 @@ def _merge(fam, groups, device, reps, cache, report, verbose):
     for g in groups:
         if isinstance(g, (ops.StreamGemmGroup, ops.GemmGroup)):
             k = (type(g).__name__, getattr(g, "dtype", None), getattr(g, "ak", None), getattr(g, "bk", None),
-                 _shape_of(g))
+                 getattr(g, "fp8", None), _shape_of(g))
             by.setdefault(k, []).append(g)
     out = [g for g in groups if not isinstance(g, (ops.StreamGemmGroup, ops.GemmGroup))]
     for k, gs in by.items():
@@ -94,7 +94,8 @@ def _merge(fam, groups, device, reps, cache, report, verbose):
             continue
         descs = [d for g in gs for d in g.descs]
         shape = _shape_of(gs[0])
-        make = ((lambda: ops.StreamGemmGroup(descs, bm=shape[0], bn=shape[1], grid=gs[0].grid, device=device))
+        make = ((lambda: ops.StreamGemmGroup(descs, bm=shape[0], bn=shape[1], grid=gs[0].grid, device=device,
+                                             fp8=gs[0].fp8))
                 if isinstance(gs[0], ops.StreamGemmGroup) else
                 (lambda: ops.GemmGroup(descs, gs[0].dtype, shape, device)))
         key = "%s|merge|%s" % (_group_key(fam, gs[0]), sorted((d.M, d.N, d.K, d.batch) for d in descs))

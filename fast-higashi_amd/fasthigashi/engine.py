@@ -81,6 +81,8 @@ def shape_for_n(N: int) -> int:
 # bf16 path: positions of the k-contiguous P^T and X^T copies are padded to a multiple of
 # KPAD elements, so every 64-element k-chunk row of a streamed tile is one aligned 128-byte line
 KPAD = 64
+# fp8 path: 128 one-byte positions per 128-byte line
+KPAD8 = 128
 
 
 def shape_for_m(M: int) -> int:
@@ -242,8 +244,10 @@ class SweepEngine:
     def __init__(self, tiles: Sequence[Sequence[TileSpec]], n_bins: Sequence[int], sizes: Sequence[int],
                  n_cells: int, rank: int, dtype: str = "fp32", device="cuda", names=None,
                  ns_iter: int = 48, ns_tol: float = 1e-20, meta_ksplit: int = 32, n_iter_max_hist: int = 512):
-        assert dtype in ("fp32", "bf16")
+        assert dtype in ("fp32", "bf16", "fp8")
         self.dtype = dtype
+        self.f8 = dtype == "fp8"
+        # X as built (fp8: bf16, quantised to e4m3 by finish_X, which then frees it)
         self.xdt = F32 if dtype == "fp32" else BF16
         self.tdtype = torch.float32 if dtype == "fp32" else torch.bfloat16
         self.device = torch.device(device)
@@ -296,18 +300,37 @@ class SweepEngine:
         # P_i^T sp x positions, positions padded to a multiple of KPAD)
         self.P16 = None
         self.Xt = self.P16t = None
-        if dtype == "bf16":
+        self.X8 = self.X8t = self.P8t = None
+        if dtype in ("bf16", "fp8"):
+            kp = KPAD8 if self.f8 else KPAD
             xt = pt = 0
             for cl in self.chroms:
                 cl.xtoff, cl.ptoff = [], []
                 for t in cl.tiles:
-                    wp = rup(t.width, KPAD)
+                    wp = rup(t.width, kp)
                     cl.xtoff.append(xt)
                     cl.ptoff.append(pt)
                     xt += t.rows * self.Npad * wp
                     pt += t.rows * cl.sp * wp
-            self.Xt = z(xt, dt=torch.bfloat16)
             self.P16t = z(pt, dt=torch.bfloat16)
+            if self.f8:
+                # e4m3 copies: X (positions x cells) for Y = X C, X_i^T per bin for T_hat = P^T X;
+                # one scale per row (position) of X, per row of C^T and per row of P^T
+                self.X8 = z(self.Wtot, self.Npad, dt=torch.uint8)
+                self.X8t = z(xt, dt=torch.uint8)
+                self.P8t = z(pt, dt=torch.uint8)
+                self.xrow_scale = z(self.Wtot)
+                self.Ct8 = z(self.Dld, self.Npad, dt=torch.uint8)
+                self.ctscale = z(self.Dld)
+                prow = 0
+                for cl in self.chroms:
+                    cl.psoff = []
+                    for t in cl.tiles:
+                        cl.psoff.append(prow)
+                        prow += t.rows * cl.sp
+                self.pscale = z(max(prow, 1))
+            else:
+                self.Xt = z(xt, dt=torch.bfloat16)
         self.That = z(self.Ttot, self.Npad, dt=self.tdtype)
         self.Lk = z(loff, dt=self.tdtype)
         self.Zt = z(self.Npad, self.Dld)
@@ -315,7 +338,7 @@ class SweepEngine:
         self.SVDt = z(self.Npad, self.R)
         self.GMpart = z(meta_ksplit * self.R * self.R, dt=torch.float64)
         # bf16 path: M^T (R x cells, cells contiguous) for the streamed T_tilde = T_hat M
-        self.M16t = z(self.R, self.Npad, dt=torch.bfloat16) if dtype == "bf16" else None
+        self.M16t = z(self.R, self.Npad, dt=torch.bfloat16) if dtype in ("bf16", "fp8") else None
         self.Ttil = z(ttoff)
         # polar batches: per-bin (one group per chromosome) and the meta-embedding
         self.pb = ops.PolarBatch([c.s for c in self.chroms], [c.n for c in self.chroms], ns_iter, ns_tol, dev)
@@ -351,6 +374,8 @@ class SweepEngine:
         return self.D_all[:, cl.doff: cl.doff + cl.sp]
 
     def X_tile(self, c: int, t: int) -> torch.Tensor:
+        if self.X is None:
+            raise RuntimeError("X was quantised and released (fp8 path)")
         cl = self.chroms[c]
         tl = cl.tiles[t]
         r0 = cl.xrow + cl.trow[t]
@@ -395,7 +420,11 @@ class SweepEngine:
 
     def finish_X(self):
         """Squared norms of each chromosome's X (the reference's first-sweep ||X||^2); on the
-        bf16 path also the per-bin transposed copy X_i^T used by T_hat = P^T X."""
+        bf16 path also the per-bin transposed copy X_i^T used by T_hat = P^T X; on the fp8 path
+        the e4m3 copies X8 and X8t (one scale per position row), after which the bf16 X is
+        released."""
+        if self.X is None:
+            raise RuntimeError("X was already quantised and released (fp8 path)")
         if self.Xt is not None:
             self.g_xt.run()
         rpb = 64
@@ -408,13 +437,20 @@ class SweepEngine:
             off = torch.tensor([0, nb], dtype=torch.int64, device=self.device)
             L.call("fh_segment_sum", part.data_ptr(), off.data_ptr(), 1, self.tnorm.data_ptr() + 8 * c, 0,
                    ops.stream_handle())
+        if self.f8:
+            self.g_x8.run()
+            self.g_x8t.run()
+            torch.cuda.synchronize()
+            self.X = None
+            torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ plan
     def _build(self):
         dev = self.device
         R, Np, Dld = self.R, self.Npad, self.Dld
-        xe = 2 if self.dtype == "bf16" else 4
-        out16 = L.GEMM_OUT_BF16 if self.dtype == "bf16" else 0
+        lowp = self.dtype in ("bf16", "fp8")
+        xe = 2 if lowp else 4
+        out16 = L.GEMM_OUT_BF16 if lowp else 0
         Xp, Ctp, Thp, Lp = self.X.data_ptr(), self.Ct.data_ptr(), self.That.data_ptr(), self.Lk.data_ptr()
         f4 = lambda t, o: t.data_ptr() + 4 * o
         # (1) Ct_c = (M D_c)^T, computed as M D_c with a transposed store
@@ -430,7 +466,26 @@ class SweepEngine:
         for cl in self.chroms:
             d2.append(make_desc(Xp + xe * cl.xrow * Np, Ctp + xe * cl.doff * Np, f4(self.Y, cl.yoff),
                                 cl.W, cl.s, Np, Np, 1, 1, Np, cl.sp, 1))
-        if self.dtype == "bf16":
+        if self.f8:
+            # e4m3 X (scale per position row) against e4m3 C_c^T (scale per column r of C); both
+            # scales applied in the epilogue.  X itself is quantised once, by finish_X.
+            q = []
+            d2 = []
+            self.g_x8 = ops.QuantRowsGroup([(Xp + 2 * cl.xrow * Np, BF16, Np, cl.W, Np,
+                                             self.X8.data_ptr() + cl.xrow * Np, Np, f4(self.xrow_scale, cl.xrow), 0)
+                                            for cl in self.chroms], dev)
+            for c, cl in enumerate(self.chroms):
+                q.append((Ctp + 2 * cl.doff * Np, BF16, Np, cl.s, Np, self.Ct8.data_ptr() + cl.doff * Np, Np,
+                          f4(self.ctscale, cl.doff), 0))
+                d2.append(make_desc(self.X8.data_ptr() + cl.xrow * Np, self.Ct8.data_ptr() + cl.doff * Np,
+                                    f4(self.Y, cl.yoff), cl.W, cl.s, Np, Np, 1, 1, Np, cl.sp, 1,
+                                    scale_a=f4(self.xrow_scale, cl.xrow), scale_b=f4(self.ctscale, cl.doff)))
+            self.g_ct8 = ops.QuantRowsGroup(q, dev)
+            narrow = [d for d in d2 if d.N <= 128]
+            wide = [d for d in d2 if d.N > 128]
+            self.g_y = StreamMulti([ops.StreamGemmGroup(v, bm=128, bn=bn, device=dev, fp8=True)
+                                    for v, bn in ((narrow, 128), (wide, 256)) if v])
+        elif self.dtype == "bf16":
             narrow = [d for d in d2 if d.N <= 128]
             wide = [d for d in d2 if d.N > 128]
             self.g_y = StreamMulti([ops.StreamGemmGroup(v, bm=128, bn=bn, device=dev)
@@ -479,7 +534,32 @@ class SweepEngine:
         # (8) T_hat_i = P_i^T X_i.  bf16: both operands k-contiguous (P_i^T and X_i^T copies);
         # fp32: P_i and X_i as stored (m- and n-contiguous)
         d8 = []
-        if self.dtype == "bf16":
+        if self.f8:
+            # P_i^T: fp32 -> bf16 transposed copy -> times the X row scales of the bin's positions ->
+            # e4m3 with a scale per row (p); X_i^T: byte transpose of X8; T_hat = scale_P[p] (P8^T X8)
+            x8t, pt, q = [], [], []
+            for c, cl in enumerate(self.chroms):
+                for t, tl in enumerate(cl.tiles):
+                    wp = rup(tl.width, KPAD8)
+                    xo = (cl.xrow + cl.trow[t]) * Np
+                    x8t.append((self.X8.data_ptr() + xo, L.FP8, self.X8t.data_ptr() + cl.xtoff[t], L.FP8, tl.width,
+                                Np, Np, wp, tl.rows, tl.width * Np, Np * wp))
+                    po = cl.yoff + cl.trow[t] * cl.sp
+                    pt.append((f4(self.P, po), F32, self.P16t.data_ptr() + 2 * cl.ptoff[t], BF16, tl.width,
+                               cl.s, cl.sp, wp, tl.rows, tl.width * cl.sp, cl.sp * wp))
+                    # P_i^T scaled by the row factors of X_i (positions), one scale per row p
+                    q.append((self.P16t.data_ptr() + 2 * cl.ptoff[t], BF16, wp, tl.rows * cl.sp, tl.width,
+                              self.P8t.data_ptr() + cl.ptoff[t], wp, f4(self.pscale, cl.psoff[t]),
+                              f4(self.xrow_scale, cl.xrow + cl.trow[t]), cl.sp, tl.width))
+                    d8.append(make_desc(self.P8t.data_ptr() + cl.ptoff[t], self.X8t.data_ptr() + cl.xtoff[t],
+                                        Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
+                                        cl.s, Np, tl.width, wp, 1, 1, wp, Np, 1, tl.rows, cl.sp * wp,
+                                        Np * wp, cl.sp * Np, flags=out16, scale_a=f4(self.pscale, cl.psoff[t]),
+                                        sSa=cl.sp))
+            self.g_x8t = ops.TransposeGroup(x8t, dev)
+            self.g_pt = ops.TransposeGroup(pt, dev)
+            self.g_p8 = ops.QuantRowsGroup(q, dev)
+        elif self.dtype == "bf16":
             xt = []
             pt = []
             for c, cl in enumerate(self.chroms):
@@ -506,12 +586,13 @@ class SweepEngine:
                                         Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
                                         cl.s, Np, tl.width, 1, cl.sp, Np, 1, Np, 1, tl.rows, tl.width * cl.sp,
                                         tl.width * Np, cl.sp * Np, flags=out16))
-        if self.dtype == "bf16":
+        if lowp:
             # streaming kernel: one persistent launch per tile height (64 .. 160 rows >= s)
             by = {}
             for d in d8:
                 by.setdefault(ops.stream_bm(d.M), []).append(d)
-            self.g_that = StreamMulti([ops.StreamGemmGroup(v, bm=k, device=dev) for k, v in sorted(by.items())])
+            self.g_that = StreamMulti([ops.StreamGemmGroup(v, bm=k, device=dev, fp8=self.f8)
+                                       for k, v in sorted(by.items())])
         else:
             self.g_that = MultiGemm(d8, self.xdt, lambda d: shape_for_m(d.M), dev)
         # (10) Zt_c = T_hat_c^T L_c^T
@@ -539,7 +620,7 @@ class SweepEngine:
         # (14) T_tilde_c = T_hat_c M.  bf16: streaming kernel over T_hat (k = cells contiguous)
         # against M^T, one 128 x 256 tile column for R <= 256, so T_hat is read once
         d14 = []
-        if self.dtype == "bf16":
+        if lowp:
             self.g_mt = ops.TransposeGroup([(self.M.data_ptr(), F32, self.M16t.data_ptr(), BF16, Np, R, R, Np,
                                              1, 0, 0)], dev)
             for cl in self.chroms:
@@ -675,6 +756,8 @@ class SweepEngine:
         # pass 1
         mk("ct")
         self.g_ct.run(stream)
+        if self.f8:
+            self.g_ct8.run(stream)
         self._run_marked("y", self.g_y, stream)
         mk("scale_temp")
         L.call("fh_scale_rows", self.rs_tab.data_ptr(), self.rs_n, self.rs_max, st)
@@ -686,8 +769,10 @@ class SweepEngine:
         mk("polar_apply")
         self.g_p.run(stream)
         L.call("fh_segment_sum", self.pb.nuc.data_ptr(), self.seg.data_ptr(), self.K, self.xU.data_ptr(), 0, st)
-        if self.dtype == "bf16":
+        if self.dtype in ("bf16", "fp8"):
             self.g_pt.run(stream)
+        if self.f8:
+            self.g_p8.run(stream)
         # pass 2
         self._run_marked("that", self.g_that, stream)
         mk("krp")
@@ -705,7 +790,7 @@ class SweepEngine:
                self.Gm.data_ptr(), st)
         self.pbm.solve(stream)
         self.g_m.run(stream)
-        if self.dtype == "bf16":
+        if self.dtype in ("bf16", "fp8"):
             self.g_mt.run(stream)
         # pass 3
         self._run_marked("ttilde", self.g_ttil, stream)
@@ -715,8 +800,10 @@ class SweepEngine:
         projections and the current A, B, D, then its polar factor.  Leaves self.M unchanged."""
         st = ops.stream_handle(stream)
         M_keep = self.M.clone()
-        if self.dtype == "bf16":
+        if self.dtype in ("bf16", "fp8"):
             self.g_pt.run(stream)
+        if self.f8:
+            self.g_p8.run(stream)
         self.g_that.run(stream)
         for c, cl in enumerate(self.chroms):
             L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,

@@ -46,7 +46,8 @@ def pick_shape(M: int, N: int) -> int:
 def make_desc(A: int, B: int, C: int, M: int, N: int, K: int, sAm: int, sAk: int, sBk: int, sBn: int,
               sCm: int, sCn: int, batch: int = 1, sAb: int = 0, sBb: int = 0, sCb: int = 0,
               alpha: float = 1.0, beta: float = 0.0, diag: float = 0.0, flags: int = 0, ksplit: int = 1,
-              skip: int = 0, scale_a: int = 0, scale_b: int = 0, sCk: int = 0) -> L.GemmDesc:
+              skip: int = 0, scale_a: int = 0, scale_b: int = 0, sCk: int = 0, sSa: int = 0,
+              sSb: int = 0) -> L.GemmDesc:
     d = L.GemmDesc()
     d.A, d.B, d.C = A, B, C
     d.M, d.N, d.K = M, N, K
@@ -56,6 +57,7 @@ def make_desc(A: int, B: int, C: int, M: int, N: int, K: int, sAm: int, sAk: int
     d.flags = flags | ((ksplit if ksplit > 1 else 0) << 8)
     d.alpha, d.beta, d.diag = alpha, beta, diag
     d.scale_a, d.scale_b, d.skip = scale_a or None, scale_b or None, skip or None
+    d.sSa, d.sSb = sSa, sSb
     return d
 
 
@@ -79,6 +81,7 @@ class TransposeGroup:
             d.src, d.dst, d.sdt, d.ddt = sp_, dp, sdt, ddt
             d.rows, d.cols, d.lds, d.ldd, d.batch, d.sbs, d.sbd = rows, cols, lds, ldd, batch, sbs, sbd
             assert cols <= lds and rows <= ldd, "transpose: leading dims shorter than the extents"
+            assert (sdt == L.FP8) == (ddt == L.FP8), "transpose: fp8 bytes only to fp8"
             d.tiles_r, d.tiles_c = -(-rows // 64), -(-cols // 64)
             d.tile_begin = total
             total += batch * d.tiles_r * d.tiles_c
@@ -155,17 +158,20 @@ class StreamGemmGroup:
     workgroups, global->LDS ring); tiles bm x bn: bm in STREAM_BM with bn = 128, or 128 x 256."""
 
     def __init__(self, descs: Sequence[L.GemmDesc], bm: Optional[int] = None, grid: int = 0, device="cuda",
-                 bn: int = 128):
+                 bn: int = 128, fp8: bool = False):
         assert len(descs) > 0
+        self.fp8 = fp8
+        v = 16 if fp8 else 8             # elements per 16-byte chunk
         for d in descs:
             assert d.sAk == 1 and d.sBk == 1, "streaming GEMM needs k-contiguous A and B"
-            # 16-byte chunks along k: rows readable (and zero) up to K rounded to 8
+            # 16-byte chunks along k: rows readable (and zero) up to K rounded to a whole chunk
             for ld, bs, p, nm in ((d.sAm, d.sAb, d.A, "A"), (d.sBn, d.sBb, d.B, "B")):
-                assert ld % 8 == 0 and ld >= -(-d.K // 8) * 8, "%s leading dim %d (K %d)" % (nm, ld, d.K)
-                assert d.batch == 1 or bs % 8 == 0, "%s batch stride %d not a multiple of 8" % (nm, bs)
+                assert ld % v == 0 and ld >= -(-d.K // v) * v, "%s leading dim %d (K %d)" % (nm, ld, d.K)
+                assert d.batch == 1 or bs % v == 0, "%s batch stride %d not a multiple of %d" % (nm, bs, v)
                 assert p % 16 == 0, "%s pointer not 16-byte aligned" % nm
             assert d.beta == 0.0 and d.diag == 0.0 and (d.flags & ~L.GEMM_OUT_BF16) == 0, \
                 "streaming GEMM: flags 0 or OUT_BF16, no beta/diag"
+            assert fp8 or not (d.scale_a or d.scale_b), "row scales only on the fp8 streaming GEMM"
         self.bm = bm if bm is not None else stream_bm(max(d.M for d in descs))
         self.bn = bn
         assert (bn == 128 and self.bm in STREAM_BM) or (bn, self.bm) == (256, 128), "tile %dx%d" % (self.bm, bn)
@@ -177,8 +183,33 @@ class StreamGemmGroup:
         self.table = _upload(arr, device)
 
     def run(self, stream=None):
-        L.call("fh_gemm_stream_launch", self.bm, self.bn, self.table.data_ptr(), self.n, self.total, self.grid,
-               stream_handle(stream))
+        L.call("fh_gemm_stream8_launch" if self.fp8 else "fh_gemm_stream_launch", self.bm, self.bn,
+               self.table.data_ptr(), self.n, self.total, self.grid, stream_handle(stream))
+
+
+class QuantRowsGroup:
+    """One launch of fh_quant_rows over a table: items (src, sdt, lds, rows, cols, dst, ldd, scale,
+    colscale or 0[, cs_rows, cs_stride]); one e4m3 scale per row, row r multiplied by
+    colscale + (r // cs_rows) * cs_stride before its amax."""
+
+    def __init__(self, items, device="cuda"):
+        tab = []
+        for it in items:
+            (src, sdt, lds, rows, cols, dst, ldd, scale, colscale) = it[:9]
+            cs_rows, cs_stride = (it[9], it[10]) if len(it) > 9 else (0, 0)
+            assert cols <= lds and cols <= ldd and ldd % 16 == 0, "quant rows: extents"
+            q = L.QuantRows()
+            q.src, q.sdt, q.lds, q.rows, q.cols = src, sdt, lds, rows, cols
+            q.dst, q.ldd, q.scale, q.colscale = dst, ldd, scale, colscale or None
+            q.cs_rows, q.cs_stride = cs_rows, cs_stride
+            tab.append(q)
+        arr = (L.QuantRows * len(tab))(*tab)
+        self.total = L.lib().fh_quant_rows_prepare(ctypes.cast(arr, ctypes.c_void_p), len(tab))
+        self.n = len(tab)
+        self.table = _upload(arr, device)
+
+    def run(self, stream=None):
+        L.call("fh_quant_rows", self.table.data_ptr(), self.n, self.total, stream_handle(stream))
 
 
 class Gemm64Group:

@@ -75,6 +75,7 @@ typedef struct FhGemmDesc {
   const int* skip;        /* nullable device flag: when *skip != 0 the descriptor is a no-op */
   int64_t tile_begin;     /* written by fh_gemm_prepare */
   int32_t tiles_m, tiles_n;
+  int64_t sSa, sSb;       /* fp8 streaming GEMM: batch strides of the per-row scales scale_a / scale_b */
 } FhGemmDesc;
 
 enum {
@@ -106,6 +107,15 @@ int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhGemmDesc* de
  * when K % 8 != 0; flags 0 or FH_GEMM_OUT_BF16, beta 0. */
 int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                           int grid, fh_stream_t stream);
+/* The same streaming GEMM with FP8 (OCP e4m3) operands: MFMA 16x16x128 f8f6f4, fp32 accumulate,
+ * k-steps of 128 elements (one 128-byte row chunk, as for bf16).  C[b][m][n] = alpha *
+ * scale_a[b * sSa + m] * scale_b[b * sSb + n] * sum_k A[b][m][k] B[b][n][k], scale_a / scale_b
+ * (fp32, per row) nullable.  Contract as above with elements of one byte: leading dims and batch
+ * strides multiples of 16, rows zero from K to the next multiple of 16.  The MTTKRP contractions
+ * Y_c = X_c C_c (parafac2_intergrative.py:374-383) and T_hat_i = P_i^T X_i (:422-430) of the FP8
+ * path. */
+int fh_gemm_stream8_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
+                           int grid, fh_stream_t stream);
 
 /* ------------------------------------------------------------------ polar factor (FP64)
  * P = Y G^{-1/2} with G = Y^T Y, and the nuclear norm tr(G^{1/2}), by the coupled
@@ -258,7 +268,8 @@ int fh_sweep_errors(const double* tnorm, const double* core, const double* xU, c
 /* out[i] = sum_b in[b*stride + i] in fixed order (split-K reductions, e.g. SVD_term :430) */
 int fh_batch_sum(const void* in, int dt, int64_t nb, int64_t stride, int64_t count, void* out,
                  fh_stream_t stream);
-/* grouped batched transpose dst[b][c][r] = src[b][r][c] with fp32 / bf16 conversion, one launch for
+/* grouped batched transpose dst[b][c][r] = src[b][r][c] with fp32 / bf16 conversion (or FH_FP8 to FH_FP8:
+ * bytes copied as they are), one launch for
  * a table of descriptors (tile_begin: prefix sum of batch * tiles_r * tiles_c, 64 x 64 tiles).
  * Keeps both operands of the P^T X product k-contiguous (the positions x cells tiles of
  * fetch(transpose=True), sparse_for_schic.py:588-613, and the projections of
@@ -274,6 +285,35 @@ typedef struct FhTransDesc {
 int fh_transpose_grouped(const FhTransDesc* descs_dev, int n_desc, int64_t total_tiles, fh_stream_t stream);
 /* X[b][i][i] = v (add = 0) or += v (add = 1): torch.eye / diagonal add, partial_rwr.py:113, 118 */
 int fh_diag(float* X, int64_t nb, int64_t stride, int64_t ld, int64_t n, float v, int add, fh_stream_t stream);
+
+/* ------------------------------------------------------------------ FP8 quantisation (no counterpart:
+ * the reference computes in FP32).  OCP e4m3, scale = amax / 448 per group, round to nearest
+ * even; destination columns from cols to the leading dimension are written as zero. */
+/* amax[r] = max_c |X[r][c]| of a rows x cols fp32 / bf16 matrix */
+int fh_amax_rows(const void* X, int dt, int64_t ld, int64_t rows, int64_t cols, float* amax, fh_stream_t stream);
+/* per chunk of `chunk` columns: amax[ceil(cols / chunk)] of X[r][c] / row_div[r] (row_div nullable; a
+ * zero divisor counts as 1) */
+int fh_amax_col_chunks(const void* X, int dt, int64_t ld, int64_t rows, int64_t cols, int64_t chunk,
+                       const float* row_div, float* amax, fh_stream_t stream);
+/* dst = e4m3(X / row_div[r] / scale(chunk of the column)), scale = amax / 448 (1 if 0);
+ * col_scale[c] = scale of column c's chunk (nullable) */
+int fh_quant_col_chunks(const void* X, int dt, int64_t ld, int64_t rows, int64_t cols, int64_t chunk,
+                        const float* row_div, const float* amax, uint8_t* dst, int64_t ldd, float* col_scale,
+                        fh_stream_t stream);
+/* a table of matrices quantised with one scale per row (amax over the row of x * colscale[c]) */
+typedef struct FhQuantRows {
+  const void* src;        /* rows x cols, leading dim lds, dtype sdt (FH_F32 / FH_BF16) */
+  float* scale;           /* rows: the row scales */
+  uint8_t* dst;           /* rows x ldd e4m3 */
+  const float* colscale;  /* cols multipliers applied before the amax (nullable); row r uses
+                             colscale + (r / cs_rows) * cs_stride */
+  int64_t rows, cols, lds, ldd;
+  int64_t cs_rows, cs_stride;   /* cs_rows 0: one set of multipliers for all rows */
+  int32_t sdt, pad;
+  int64_t row_begin;      /* filled by fh_quant_rows_prepare */
+} FhQuantRows;
+int64_t fh_quant_rows_prepare(FhQuantRows* tab_host, int n);
+int fh_quant_rows(const FhQuantRows* tab_dev, int n_tab, int64_t total_rows, fh_stream_t stream);
 
 /* ------------------------------------------------------------------ reconstruction loss
  * L = sum_c sum_i ||X_i - P_i B_c diag(A_ci) C_c^T||^2, C_c = M D_c, P held fixed.  No

@@ -342,3 +342,113 @@ def test_gemm_stream_matches_tiled_gemm(cuda):
     torch.cuda.synchronize()
     diff = (C[:, :M] - C2[:, :M]).abs().max().item() / C2.abs().max().item()
     assert diff < 1e-5, diff
+
+
+def _stream8_case(cuda, gen, M, N, K, batch, out16, scaled):
+    """fp8 (e4m3) operands in the streaming layout (k padded to a multiple of 16 with zeros),
+    optional per-row scales of A (batch stride sp) and of B (shared by the batches)."""
+    from fasthigashi import ops, _lib as L
+    wp = -(-K // 16) * 16
+    sp = -(-M // 4) * 4
+    A = torch.zeros(batch, sp, wp)
+    B = torch.zeros(batch, N, wp)
+    A[:, :M, :K] = torch.randn(batch, M, K, generator=gen) * 40
+    B[:, :, :K] = torch.randn(batch, N, K, generator=gen) * 3
+    A8 = A.to(torch.float8_e4m3fn)
+    B8 = B.to(torch.float8_e4m3fn)
+    sa = (torch.rand(batch, sp, generator=gen) + 0.5) if scaled else None
+    sb = (torch.rand(N, generator=gen) + 0.5) if scaled else None
+    Ad, Bd = A8.view(torch.uint8).to(cuda), B8.view(torch.uint8).to(cuda)
+    sad = sa.to(cuda) if scaled else None
+    sbd = sb.to(cuda) if scaled else None
+    C = torch.full((batch, sp, N), 7.0, dtype=torch.bfloat16 if out16 else torch.float32, device=cuda)
+    d = ops.make_desc(Ad.data_ptr(), Bd.data_ptr(), C.data_ptr(), M, N, K, wp, 1, 1, wp, N, 1, batch,
+                      sp * wp, N * wp, sp * N, alpha=0.5, flags=L.GEMM_OUT_BF16 if out16 else 0,
+                      scale_a=sad.data_ptr() if scaled else 0, sSa=sp, scale_b=sbd.data_ptr() if scaled else 0)
+    ref = 0.5 * (A8[:, :M, :K].double() @ B8[:, :, :K].double().transpose(1, 2))
+    if scaled:
+        ref = ref * sa[:, :M, None].double() * sb[None, None, :].double()
+    return d, C, ref, M, (Ad, Bd, sad, sbd)
+
+
+@pytest.mark.parametrize("bmbn", [(64, 128), (96, 128), (128, 128), (160, 128), (128, 256)])
+def test_gemm_stream8_matches_reference(cuda, bmbn):
+    """fp8 streaming GEMM (MFMA 16x16x128 f8f6f4, e4m3 operands, fp32 accumulate) against the
+    product of the decoded e4m3 values in fp64; K below, at and above one 128-wide step and not a
+    multiple of 16, ragged N, batches, with and without per-row scales, fp32 and bf16 outputs.
+    The MFMA's internal sum of the 128 products per step is not an exact fp32 sum: errors of a
+    few 1e-5 of the output scale are its rounding, far below the e4m3 operand rounding (2^-4)."""
+    from fasthigashi import ops
+    gen = torch.Generator().manual_seed(31)
+    cases = [(28, 300, 40, 3, False, True), (150, 1000, 200, 2, False, False), (97, 129, 128, 4, True, True),
+             (128, 256, 130, 1, False, True), (60, 77, 13, 5, True, False), (150, 1000, 300, 2, True, True),
+             (33, 264, 256, 3, False, True)]
+    built = [_stream8_case(cuda, gen, M, N, K, b, o, s) for (M, N, K, b, o, s) in cases]
+    ops.StreamGemmGroup([x[0] for x in built], bm=bmbn[0], bn=bmbn[1], device=cuda, fp8=True).run()
+    torch.cuda.synchronize()
+    errs = []
+    for (d, C, ref, M, _), case in zip(built, cases):
+        got = C[:, :M].double().cpu()
+        scale = ref.abs().max().item()
+        errs.append((case, (got - ref).abs().max().item() / scale))
+        assert bool((C[:, M:].float() == 7.0).all()), (bmbn, case)
+    print("stream8 %s: %s" % (bmbn, ", ".join("%.2e" % e for _, e in errs)), flush=True)
+    for case, err in errs:
+        assert err < (1e-2 if case[4] else 2e-4), (bmbn, case, err)
+
+
+def test_quantisation_kernels_match_torch_e4m3(cuda):
+    """fh_amax_rows, fh_amax_col_chunks / fh_quant_col_chunks (rows divided by their amax, then a
+    scale per chunk of 64 columns) and fh_quant_rows (scale per row, with per-row-block column
+    multipliers) against torch's float8_e4m3fn conversion of the same scaled values: equal bytes;
+    columns past cols zero."""
+    from fasthigashi import ops, _lib as L
+    gen = torch.Generator().manual_seed(37)
+    rows, cols, ld, ldd, ch = 300, 200, 208, 224, 64
+    X = torch.zeros(rows, ld)
+    X[:, :cols] = torch.randn(rows, cols, generator=gen) * torch.logspace(-3, 2, cols)[None, :]
+    X[:, 130:140] = 0.0                                           # an all-zero run inside a chunk
+    X[:, 192:200] = 0.0
+    Xb = X.to(torch.bfloat16).to(cuda)
+    nch = -(-cols // ch)
+    amax = torch.zeros(nch, device=cuda)
+    dst = torch.full((rows, ldd), 0x55, dtype=torch.uint8, device=cuda)
+    csc = torch.zeros(cols, device=cuda)
+    rdiv = torch.zeros(rows, device=cuda)
+    st = ops.stream_handle()
+    L.call("fh_amax_rows", Xb.data_ptr(), L.BF16, ld, rows, cols, rdiv.data_ptr(), st)
+    L.call("fh_amax_col_chunks", Xb.data_ptr(), L.BF16, ld, rows, cols, ch, rdiv.data_ptr(), amax.data_ptr(), st)
+    L.call("fh_quant_col_chunks", Xb.data_ptr(), L.BF16, ld, rows, cols, ch, rdiv.data_ptr(), amax.data_ptr(),
+           dst.data_ptr(), ldd, csc.data_ptr(), st)
+    torch.cuda.synchronize()
+    x0 = Xb.float().cpu()[:, :cols]
+    ra = x0.abs().amax(1)
+    assert torch.equal(rdiv.cpu(), ra)
+    xf = x0 * torch.where(ra > 0, 1.0 / ra, torch.ones_like(ra))[:, None]
+    am = torch.stack([xf[:, k * ch:(k + 1) * ch].abs().max() for k in range(nch)])
+    assert torch.equal(amax.cpu(), am)
+    sc = torch.where(am > 0, am / 448.0, torch.ones_like(am))
+    colsc = sc.repeat_interleave(ch)[:cols]
+    assert torch.equal(csc.cpu(), colsc)
+    ref = (xf / colsc[None, :]).to(torch.float8_e4m3fn).view(torch.uint8)
+    got = dst.cpu()
+    assert torch.equal(got[:, :cols], ref), int((got[:, :cols] != ref).sum())
+    assert bool((got[:, cols:] == 0).all())
+    # rows, with column multipliers per block of 100 rows (3 blocks, multiplier sets 37 apart)
+    mult = (torch.rand(cols + 2 * 37, generator=gen) + 0.5).to(cuda)
+    dst2 = torch.full((rows, ldd), 0x55, dtype=torch.uint8, device=cuda)
+    rsc = torch.zeros(rows, device=cuda)
+    g = ops.QuantRowsGroup([(Xb.data_ptr(), L.BF16, ld, rows, cols, dst2.data_ptr(), ldd, rsc.data_ptr(),
+                             mult.data_ptr(), 100, 37)], cuda)
+    g.run()
+    torch.cuda.synchronize()
+    mc = mult.cpu()
+    mrows = torch.stack([mc[(r // 100) * 37:(r // 100) * 37 + cols] for r in range(rows)])
+    v = x0 * mrows
+    ra = v.abs().amax(1)
+    rs = torch.where(ra > 0, ra / 448.0, torch.ones_like(ra))
+    assert torch.equal(rsc.cpu(), rs)
+    ref2 = (v / rs[:, None]).to(torch.float8_e4m3fn).view(torch.uint8)
+    got2 = dst2.cpu()
+    assert torch.equal(got2[:, :cols], ref2), int((got2[:, :cols] != ref2).sum())
+    assert bool((got2[:, cols:] == 0).all())
