@@ -139,13 +139,13 @@ def build(args):
     return wl, coos, tiles, sizes, n_i, (A, B, D, M), engines
 
 
-def time_path(eng, steps, warmup, autotune=True):
+def time_path(eng, steps, warmup, autotune=True, cache=None, update=False):
     import torch
     for k in range(max(warmup, 1 if autotune else 0)):
         eng.sweep()
         if k == 0 and autotune:
             t0 = time.perf_counter()
-            rep = eng.autotune()
+            rep = eng.autotune(cache_path=cache, write=update)
             torch.cuda.synchronize()
             log("%s autotune %.1fs: %s" % (eng.dtype, time.perf_counter() - t0, "; ".join(
                 "%s %s" % (fam, ",".join(str(g.get("chosen", "merged" if g.get("merged") else "kept"))
@@ -210,8 +210,13 @@ def main():
     ap.add_argument("--paths", default="fp32,bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the rule-picked tile shapes (default: time every shape after the first "
-                         "warm-up sweep and keep the fastest; results are the same bit for bit)")
+                    help="keep the rule-picked tile shapes (default: after the first warm-up sweep take the "
+                         "tuned shapes from --tune-cache, timing every shape of the groups it does not hold; "
+                         "results are the same bit for bit)")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "fast-higashi_amd", "fasthigashi",
+                                                         "tuned_mi355x.json"),
+                    help="JSON file of tile-shape choices (read; written only with --tune-update)")
+    ap.add_argument("--tune-update", action="store_true", help="write new tile-shape choices to --tune-cache")
     ap.add_argument("--cpu-cells", type=int, default=5000)
     ap.add_argument("--cpu-chrom", type=int, default=-2, help="-2: the median-size chromosome")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -241,10 +246,12 @@ def main():
         del src, dst
     res = {}
     for dt in args.paths:
-        el, phases = time_path(engines[dt], args.steps, args.warmup, not args.no_autotune)
+        el, phases = time_path(engines[dt], args.steps, args.warmup, not args.no_autotune, args.tune_cache,
+                               args.tune_update)
         thr = wl.n_cells * len(coos) * args.steps / el
         hist = engines[dt].history()
-        res[dt] = dict(elapsed=el, thr=thr, phases=phases, fit=float(hist["rec_error"][-1]))
+        res[dt] = dict(elapsed=el, thr=thr, phases=phases, fit=float(hist["rec_error"][-1]),
+                       order=engines[dt].stream_launch_order() if dt != "fp32" else [])
         log("%s: %.1f ms/sweep, %.4g cell*chrom sweeps/s, fit %.6f" % (dt, 1000 * el / args.steps, thr,
                                                                       res[dt]["fit"]))
         log("%s phases (ms/sweep): %s" % (dt, ", ".join("%s %.2f" % kv for kv in
@@ -297,7 +304,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "paths": {dt: {"cell_chrom_sweeps_per_s": r["thr"], "ms_per_sweep": 1000 * r["elapsed"] / args.steps,
-                       "fit_last": r["fit"], "phase_ms": {k: round(v, 3) for k, v in r["phases"].items()}}
+                       "fit_last": r["fit"], "phase_ms": {k: round(v, 3) for k, v in r["phases"].items()},
+                       "stream_launch_order": r["order"]}
                   for dt, r in res.items()},
     }
     if "fp32" in res and "bf16" in res:

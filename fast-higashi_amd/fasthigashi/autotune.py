@@ -25,6 +25,8 @@ import torch
 from . import ops
 
 STREAM_SHAPES = ((64, 128), (96, 128), (128, 128), (160, 128), (128, 256))
+# choices measured on MI355X for the benchmark workloads (read-only default of fit / bench.py)
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_mi355x.json")
 TILED_SHAPES = tuple(range(7))
 
 # engine attributes holding grouped launches, by family name
@@ -115,10 +117,12 @@ def _merge(fam, groups, device, reps, cache, report, verbose):
     return out
 
 
-def tune_engine(eng, reps: int = 5, families=None, cache_path: Optional[str] = None, verbose: bool = False) -> Dict:
+def tune_engine(eng, reps: int = 5, families=None, cache_path: Optional[str] = None, verbose: bool = False,
+                write: bool = True) -> Dict:
     """Time every tile shape of each grouped launch of `eng` and keep the fastest.
 
-    Run after at least one sweep (the buffers then hold real data).  Returns
+    Run after at least one sweep (the buffers then hold real data).  Choices found in the JSON
+    file `cache_path` are taken as they are; new ones are written back when `write`.  Returns
     {family: [{"group": i, "was": shape, "chosen": shape, "ms": {shape: ms}}]}."""
     cache = {}
     if cache_path and os.path.exists(cache_path):
@@ -165,7 +169,7 @@ def tune_engine(eng, reps: int = 5, families=None, cache_path: Optional[str] = N
             _set(eng, attr, new[0])
         else:
             multi.groups = _merge(fam, new, eng.device, reps, cache, report, verbose)
-    if cache_path:
+    if cache_path and write:
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     return report

@@ -667,11 +667,15 @@ class SweepEngine:
             out.update(zip(self._tags(prefix, multi), multi.groups))
         return out
 
-    def autotune(self, reps: int = 5, cache_path: Optional[str] = None, verbose: bool = False):
+    def autotune(self, reps: int = 5, cache_path: Optional[str] = None, verbose: bool = False, write: bool = True):
         """Time every tile shape of each grouped launch and keep the fastest (see autotune.py).
         Results are unchanged bit for bit; call after at least one sweep."""
         from .autotune import tune_engine
-        return tune_engine(self, reps=reps, cache_path=cache_path, verbose=verbose)
+        return tune_engine(self, reps=reps, cache_path=cache_path, verbose=verbose, write=write)
+
+    def stream_launch_order(self):
+        """phase names of the streamed launches in the order a sweep issues them"""
+        return self._tags("y", self.g_y) + self._tags("that", self.g_that) + self._tags("ttilde", self.g_ttil)
 
     def _run_marked(self, prefix, multi, stream):
         # one marker per launch, so each launch is timed on its own

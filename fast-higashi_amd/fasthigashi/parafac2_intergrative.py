@@ -39,7 +39,8 @@ class Fast_Higashi_core:
         self.svd_random_state = None
         self.path_dtype = "fp32"
         # after the first sweep, time every tile shape of the grouped launches and keep the
-        # fastest (results unchanged bit for bit); optional JSON cache of the choices
+        # fastest (results unchanged bit for bit); choices in the packaged JSON file are reused,
+        # a path here also records new ones
         self.autotune = True
         self.autotune_cache: Optional[str] = None
 
@@ -214,7 +215,8 @@ class Fast_Higashi_core:
             start_time = time.time()
             eng.sweep()
             if iteration == 0 and self.autotune:
-                eng.autotune(cache_path=self.autotune_cache)
+                from .autotune import DEFAULT_CACHE
+                eng.autotune(cache_path=self.autotune_cache or DEFAULT_CACHE, write=self.autotune_cache is not None)
             h = eng.hist[iteration].cpu().numpy() if (tol > 0 or verbose) else None
             if h is None:
                 continue
