@@ -147,9 +147,11 @@ def time_path(eng, steps, warmup, autotune=True, cache=None, update=False):
             t0 = time.perf_counter()
             rep = eng.autotune(cache_path=cache, write=update)
             torch.cuda.synchronize()
-            log("%s autotune %.1fs: %s" % (eng.dtype, time.perf_counter() - t0, "; ".join(
-                "%s %s" % (fam, ",".join(str(g.get("chosen", "merged" if g.get("merged") else "kept"))
-                                         for g in groups)) for fam, groups in rep.items())))
+            cs = rep.pop("_cache")[0]
+            log("%s autotune %.1fs (cache %s: %d hits, %d timed): %s" % (
+                eng.dtype, time.perf_counter() - t0, os.path.basename(cs["path"]), cs["hits"], cs["misses"],
+                "; ".join("%s %s" % (fam, ",".join(str(g.get("chosen", "merged" if g.get("merged") else "kept"))
+                                                   for g in groups)) for fam, groups in rep.items())))
     torch.cuda.synchronize()
     eng.timing = []
     t0 = time.perf_counter()

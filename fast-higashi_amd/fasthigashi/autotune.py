@@ -59,8 +59,13 @@ def _time_ms(run, reps: int) -> float:
     return t[len(t) // 2]
 
 
+def _device_tag() -> str:
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return "%s/%dCU" % (p.gcnArchName.split(":")[0], p.multi_processor_count)
+
+
 def _group_key(family: str, g) -> str:
-    dev = torch.cuda.get_device_name()
+    dev = _device_tag()
     kind = ("stream8" if g.fp8 else "stream") if isinstance(g, ops.StreamGemmGroup) else "tiled%d" % g.dtype
     sizes = sorted((d.M, d.N, d.K, d.batch) for d in g.descs)
     return "%s|%s|%s|%s" % (dev, family, kind, sizes)
@@ -129,6 +134,7 @@ def tune_engine(eng, reps: int = 5, families=None, cache_path: Optional[str] = N
         with open(cache_path) as f:
             cache = json.load(f)
     report = {}
+    stats = {"hits": 0, "misses": 0}
     for fam, attr in FAMILIES:
         if families is not None and fam not in families:
             continue
@@ -146,6 +152,7 @@ def tune_engine(eng, reps: int = 5, families=None, cache_path: Optional[str] = N
                 continue
             key = _group_key(fam, g)
             ms = {}
+            stats["hits" if key in cache else "misses"] += 1
             if key in cache:
                 chosen = cache[key]
                 chosen = tuple(chosen) if isinstance(chosen, list) else chosen
@@ -172,4 +179,5 @@ def tune_engine(eng, reps: int = 5, families=None, cache_path: Optional[str] = N
     if cache_path and write:
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
+    report["_cache"] = [dict(stats, path=cache_path or "")]
     return report
