@@ -65,8 +65,6 @@ __device__ __forceinline__ TilePos tile_pos(CDesc& d, int64_t t, int BM, int BN)
 // what the output of one tile needs
 struct StoreRef {
   void* C;           // element (m0, n0) of the output (batch applied)
-  const float* sa;   // fp8: row scales from m0 (nullable)
-  const float* sb;   // fp8: column scales from n0 (nullable)
   int64_t sCm, sCn, rows, cols;
   float alpha;
   int nk;
@@ -95,8 +93,6 @@ __device__ __forceinline__ StoreRef store_ref(CDesc* descs, int n_desc, int& di,
   r.cols = d.N - p.n0;
   r.alpha = (float)d.alpha;
   r.nk = (int)((d.K + EK - 1) / EK);
-  r.sa = d.scale_a ? d.scale_a + p.b * d.sSa + p.m0 : nullptr;
-  r.sb = d.scale_b ? d.scale_b + p.b * d.sSb + p.n0 : nullptr;
   r.live = true;
   const int al = r.bf16 ? 7 : 3;   // outputs per 16 bytes, minus one
   r.vec = (((uintptr_t)d.C) & 15) == 0 && d.sCn == 1 && (d.sCm & al) == 0 && (d.sCb & al) == 0;
@@ -214,7 +210,7 @@ __device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[
   if constexpr (F8) {
     // fp8: the lane's two chunks are 32 consecutive k of the 128-wide step (k = 32 (l / 16) ..),
     // read straight into the 8-register MFMA operands; one MFMA 16x16x128 per block, scale
-    // operands 127 = 2^0 (scales are applied in the epilogue)
+    // operands 127 = 2^0 (the operands' row scales are applied outside the GEMM)
     typedef int v8i __attribute__((ext_vector_type(8)));
     v8i bf8[WN_T], af8[WM_T];
 #pragma unroll
@@ -279,7 +275,7 @@ __device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[
 // lane reads back 16 bytes of row l / 8 (chunk l % 8), so one store covers 8 full 128-byte
 // rows of C.  Rows that are not 16-byte aligned, and the ragged n edge, are written element
 // by element from the staged rows (a rolled loop).
-template <int ESZ, class C, bool F8>
+template <int ESZ, class C>
 __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C::WM_T][C::WN_T], int wm, int wn,
                                            int lane, uint8_t* so) {
   constexpr int WM_T = C::WM_T, WN_T = C::WN_T;
@@ -290,21 +286,6 @@ __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C:
   const float alpha = sr.alpha;
   const int64_t sCm = sr.sCm, sCn = sr.sCn, rows = sr.rows, cols = sr.cols;
   const int half = (lane >> 3) & 1;                // which 8 of the block's 16 rows this lane holds
-  // fp8: the lane's row and column scales, loaded once per tile (indices clamped to the valid
-  // rows / columns; the clamped ones belong to outputs that are not stored)
-  float fa[WM_T];
-  floatx4 fb[WN_T];
-  if constexpr (F8) {
-#pragma unroll
-    for (int i = 0; i < WM_T; ++i)
-      fa[i] = sr.sa ? sr.sa[min((int64_t)(wm * WM_T * 16 + i * 16 + (lane & 15)), rows - 1)] * alpha : alpha;
-#pragma unroll
-    for (int j = 0; j < WN_T; ++j) {
-      const int64_t n0 = wn * WN_T * 16 + j * 16 + (lane >> 4) * 4;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) fb[j][r] = sr.sb ? sr.sb[min(n0 + r, cols - 1)] : 1.f;
-    }
-  }
 #pragma unroll
   for (int i = 0; i < WM_T; ++i) {
     const int64_t mb = wm * WM_T * 16 + i * 16;
@@ -317,9 +298,7 @@ __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C:
           if (half == hh) {
 #pragma unroll
             for (int jj = 0; jj < JPG; ++jj) {
-              floatx4 v;
-              if constexpr (F8) v = acc[i][g * JPG + jj] * fa[i] * fb[g * JPG + jj];
-              else v = acc[i][g * JPG + jj] * alpha;
+              const floatx4 v = acc[i][g * JPG + jj] * alpha;
               uint8_t* dst = so + (lane & 7) * kStageRow + (jj * 16 + (lane >> 4) * 4) * ESZ;
               if (ESZ == 2) {
                 u32x2 w;
@@ -405,8 +384,8 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
     issue((k + STAGES - 1) % STAGES);
     compute_stage<C, F8>(ring + (k % STAGES) * C::STAGE_BYTES, acc, wm, wn, lane);
     if (++ck >= sr.nk) {
-      if (sr.bf16) store_tile<2, C, F8>(sr, acc, wm, wn, lane, SO[wave]);
-      else store_tile<4, C, F8>(sr, acc, wm, wn, lane, SO[wave]);
+      if (sr.bf16) store_tile<2, C>(sr, acc, wm, wn, lane, SO[wave]);
+      else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);
 #pragma unroll
       for (int i = 0; i < WM_T; ++i)
 #pragma unroll

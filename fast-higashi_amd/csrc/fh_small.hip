@@ -57,42 +57,47 @@ __global__ void copy2d_kernel(const void* src, int sdt, int64_t srs, int64_t scs
   }
 }
 
-// Row scaling of per-bin blocks:  Y[r][j] *= A[bin0 + r / w][j]   (j < s), one tile per blockIdx.y.
-// One wave per row (the bin index once per row); with ldy, lda multiples of 4 and 16-byte
-// aligned bases each lane scales four columns with 16-byte accesses, else one column at a time.
-struct RowScale { float* Y; const float* A; int64_t nrows, w, s, ldy, lda; };
-__global__ __launch_bounds__(NT) void scale_rows_kernel(const RowScale* __restrict__ tab) {
-  const RowScale t = tab[blockIdx.y];
+// Row scaling of per-bin blocks:  Y[r][j] *= A[bin0 + r / w][j] * rowf[r] * colf[j]   (j < s; A,
+// rowf, colf nullable), one tile per blockIdx.y.  One wave per row (the bin index once per row);
+// with ldy, lda multiples of 4 and 16-byte aligned bases each lane scales four columns with
+// 16-byte accesses, else one column at a time.
+__global__ __launch_bounds__(NT) void scale_rows_kernel(const FhRowScale* __restrict__ tab) {
+  const FhRowScale t = tab[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
   const int64_t nw = (int64_t)gridDim.x * (NT / 64);
-  const bool vec = ((t.ldy | t.lda) & 3) == 0 && ((((uintptr_t)t.Y) | ((uintptr_t)t.A)) & 15) == 0;
+  const bool vec = ((t.ldy | t.lda) & 3) == 0 &&
+                   ((((uintptr_t)t.Y) | ((uintptr_t)t.A) | ((uintptr_t)t.colf)) & 15) == 0;
   for (int64_t r = wave; r < t.nrows; r += nw) {
     float* y = t.Y + r * t.ldy;
-    const float* a = t.A + (r / t.w) * t.lda;
+    const float* a = t.A ? t.A + (r / t.w) * t.lda : nullptr;
+    const float rf = t.rowf ? t.rowf[r] : 1.f;
+    const float* cf = t.colf;
     if (vec) {
       for (int64_t j = 4 * lane; j < t.s; j += 256) {
         if (j + 4 <= t.s) {
-          floatx4 v = *(floatx4*)(y + j);
-          v *= *(const floatx4*)(a + j);
+          floatx4 v = *(floatx4*)(y + j) * rf;
+          if (a) v *= *(const floatx4*)(a + j);
+          if (cf) v *= *(const floatx4*)(cf + j);
           *(floatx4*)(y + j) = v;
         } else {
-          for (int64_t e = j; e < t.s; ++e) y[e] *= a[e];
+          for (int64_t e = j; e < t.s; ++e) y[e] *= (a ? a[e] : 1.f) * rf * (cf ? cf[e] : 1.f);
         }
       }
     } else {
-      for (int64_t j = lane; j < t.s; j += 64) y[j] *= a[j];
+      for (int64_t j = lane; j < t.s; j += 64) y[j] *= (a ? a[j] : 1.f) * rf * (cf ? cf[j] : 1.f);
     }
   }
 }
 
-// Khatri-Rao rows:  L[r][i*sp + j] = A[i][r] * B[j][r]  (j < s; zero for s <= j < sp)
+// Khatri-Rao rows:  L[r][i*sp + j] = A[i][r] * B[j][r] (* colf[i*sp + j])  (j < s; zero for s <= j < sp)
 __global__ void krp_kernel(const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n, int64_t s,
-                           int64_t sp, void* L, int ldt) {
+                           int64_t sp, void* L, int ldt, const float* colf) {
   const int64_t cols = n * sp, tot = s * cols;
   for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < tot; e += (int64_t)gridDim.x * NT) {
     int64_t r = e / cols, q = e % cols, i = q / sp, j = q % sp;
     double v = (j < s) ? (double)A[i * lda + r] * (double)B[j * ldb + r] : 0.;
+    if (colf) v *= (double)colf[q];
     store_any(L, ldt, e, (double)(float)v);
   }
 }
@@ -256,14 +261,15 @@ int fh_scale_rows(const FhRowScale* tab_dev, int n_tiles, int64_t max_elems, hip
   // max_elems bounds rows x s of a tile; one wave per row, NT / 64 rows per workgroup
   int64_t gx64 = (max_elems + 4 * NT - 1) / (4 * NT);
   unsigned gx = (unsigned)(gx64 < 1 ? 1 : gx64 > 1024 ? 1024 : gx64);
-  hipLaunchKernelGGL(scale_rows_kernel, dim3(gx, n_tiles), dim3(NT), 0, st, (const RowScale*)tab_dev);
+  hipLaunchKernelGGL(scale_rows_kernel, dim3(gx, n_tiles), dim3(NT), 0, st, tab_dev);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }
 
 int fh_krp(const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n, int64_t s, int64_t sp,
-           void* L, int l_dtype, hipStream_t st) {
-  hipLaunchKernelGGL(krp_kernel, dim3(grid_for(s * n * sp)), dim3(NT), 0, st, A, lda, B, ldb, n, s, sp, L, l_dtype);
+           void* L, int l_dtype, const float* colf, hipStream_t st) {
+  hipLaunchKernelGGL(krp_kernel, dim3(grid_for(s * n * sp)), dim3(NT), 0, st, A, lda, B, ldb, n, s, sp, L, l_dtype,
+                     colf);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }

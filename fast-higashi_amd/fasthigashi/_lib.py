@@ -29,7 +29,6 @@ class GemmDesc(Structure):
         ("alpha", c_double), ("beta", c_double), ("diag", c_double),
         ("scale_a", c_void_p), ("scale_b", c_void_p), ("skip", c_void_p),
         ("tile_begin", c_int64), ("tiles_m", c_int32), ("tiles_n", c_int32),
-        ("sSa", c_int64), ("sSb", c_int64),
     ]
 
 
@@ -49,7 +48,7 @@ class SpdSys(Structure):
 
 class RowScale(Structure):
     _fields_ = [("Y", c_void_p), ("A", c_void_p), ("nrows", c_int64), ("w", c_int64),
-                ("s", c_int64), ("ldy", c_int64), ("lda", c_int64)]
+                ("s", c_int64), ("ldy", c_int64), ("lda", c_int64), ("rowf", c_void_p), ("colf", c_void_p)]
 
 
 class BandTile(Structure):
@@ -112,7 +111,7 @@ _SIGS = {
                           c_double, c_void_p]),
     "fh_scale_rows": (c_int, [c_void_p, c_int, c_int64, c_void_p]),
     "fh_krp": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int,
-                       c_void_p]),
+                       c_void_p, c_void_p]),
     "fh_sumsq_partial": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "fh_segment_sum": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "fh_sweep_errors": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),

@@ -468,7 +468,8 @@ class SweepEngine:
                                 cl.W, cl.s, Np, Np, 1, 1, Np, cl.sp, 1))
         if self.f8:
             # e4m3 X (scale per position row) against e4m3 C_c^T (scale per column r of C); both
-            # scales applied in the epilogue.  X itself is quantised once, by finish_X.
+            # scales are applied by the row scaling that follows (Y rows *= A_i).  X itself is
+            # quantised once, by finish_X.
             q = []
             d2 = []
             self.g_x8 = ops.QuantRowsGroup([(Xp + 2 * cl.xrow * Np, BF16, Np, cl.W, Np,
@@ -478,8 +479,7 @@ class SweepEngine:
                 q.append((Ctp + 2 * cl.doff * Np, BF16, Np, cl.s, Np, self.Ct8.data_ptr() + cl.doff * Np, Np,
                           f4(self.ctscale, cl.doff), 0))
                 d2.append(make_desc(self.X8.data_ptr() + cl.xrow * Np, self.Ct8.data_ptr() + cl.doff * Np,
-                                    f4(self.Y, cl.yoff), cl.W, cl.s, Np, Np, 1, 1, Np, cl.sp, 1,
-                                    scale_a=f4(self.xrow_scale, cl.xrow), scale_b=f4(self.ctscale, cl.doff)))
+                                    f4(self.Y, cl.yoff), cl.W, cl.s, Np, Np, 1, 1, Np, cl.sp, 1))
             self.g_ct8 = ops.QuantRowsGroup(q, dev)
             narrow = [d for d in d2 if d.N <= 128]
             wide = [d for d in d2 if d.N > 128]
@@ -500,6 +500,10 @@ class SweepEngine:
                 q.Y = f4(self.Y, cl.yoff + cl.trow[t] * cl.sp)
                 q.A = f4(self.A_all, cl.aoff + tl.start * cl.sp)
                 q.nrows, q.w, q.s, q.ldy, q.lda = tl.rows * tl.width, tl.width, cl.s, cl.sp, cl.sp
+                if self.f8:
+                    # e4m3 scales of Y = X C: X's per position row, C's per column r
+                    q.rowf = f4(self.xrow_scale, cl.xrow + cl.trow[t])
+                    q.colf = f4(self.ctscale, cl.doff)
                 rs.append(q)
         self.rs_tab = ops._upload((L.RowScale * len(rs))(*rs), dev)
         self.rs_n = len(rs)
@@ -536,7 +540,9 @@ class SweepEngine:
         d8 = []
         if self.f8:
             # P_i^T: fp32 -> bf16 transposed copy -> times the X row scales of the bin's positions ->
-            # e4m3 with a scale per row (p); X_i^T: byte transpose of X8; T_hat = scale_P[p] (P8^T X8)
+            # e4m3 with a scale per row (p); X_i^T: byte transpose of X8.  T_hat is stored without
+            # its row scale scale_P[(i, p)]: Z takes it in the Khatri-Rao operand L (fh_krp) and
+            # T_tilde = T_hat M in a row scaling after the product
             x8t, pt, q = [], [], []
             for c, cl in enumerate(self.chroms):
                 for t, tl in enumerate(cl.tiles):
@@ -554,8 +560,7 @@ class SweepEngine:
                     d8.append(make_desc(self.P8t.data_ptr() + cl.ptoff[t], self.X8t.data_ptr() + cl.xtoff[t],
                                         Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
                                         cl.s, Np, tl.width, wp, 1, 1, wp, Np, 1, tl.rows, cl.sp * wp,
-                                        Np * wp, cl.sp * Np, flags=out16, scale_a=f4(self.pscale, cl.psoff[t]),
-                                        sSa=cl.sp))
+                                        Np * wp, cl.sp * Np, flags=out16))
             self.g_x8t = ops.TransposeGroup(x8t, dev)
             self.g_pt = ops.TransposeGroup(pt, dev)
             self.g_p8 = ops.QuantRowsGroup(q, dev)
@@ -633,6 +638,18 @@ class SweepEngine:
                 d14.append(make_desc(Thp + xe * cl.toff * Np, self.M.data_ptr(), f4(self.Ttil, cl.ttoff),
                                      cl.n * cl.sp, R, Np, Np, 1, R, 1, R, 1))
             self.g_ttil = MultiGemm(d14, self.xdt, lambda d: 6 if d.N > 160 else shape_for_n(d.N), dev)
+        if self.f8:
+            # T_tilde rows (i, p) times the e4m3 row scale of T_hat's row
+            tt = []
+            for c, cl in enumerate(self.chroms):
+                q = L.RowScale()
+                q.Y = f4(self.Ttil, cl.ttoff)
+                q.A = None
+                q.nrows, q.w, q.s, q.ldy, q.lda = cl.n * cl.sp, 1, R, R, 0
+                q.rowf = f4(self.pscale, cl.psoff[0])
+                tt.append(q)
+            self.tt_tab = ops._upload((L.RowScale * len(tt))(*tt), dev)
+            self.tt_max = max(q.nrows * q.s for q in tt)
         self._build_cpals()
 
     def _build_cpals(self):
@@ -782,7 +799,8 @@ class SweepEngine:
         mk("krp")
         for c, cl in enumerate(self.chroms):
             L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,
-                   cl.sp, self.Lk.data_ptr() + self.Lk.element_size() * cl.loff, self.xdt, st)
+                   cl.sp, self.Lk.data_ptr() + self.Lk.element_size() * cl.loff, self.xdt,
+                   self.pscale.data_ptr() + 4 * cl.psoff[0] if self.f8 else None, st)
         mk("z")
         self.g_z.run(stream)
         mk("svd_term")
@@ -798,6 +816,8 @@ class SweepEngine:
             self.g_mt.run(stream)
         # pass 3
         self._run_marked("ttilde", self.g_ttil, stream)
+        if self.f8:
+            L.call("fh_scale_rows", self.tt_tab.data_ptr(), self.K, self.tt_max, st)
 
     def transform(self, stream=None) -> np.ndarray:
         """Final meta-embedding (parafac2_intergrative.py:742-834): SVD_term from the stored
@@ -811,7 +831,8 @@ class SweepEngine:
         self.g_that.run(stream)
         for c, cl in enumerate(self.chroms):
             L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,
-                   cl.sp, self.Lk.data_ptr() + self.Lk.element_size() * cl.loff, self.xdt, st)
+                   cl.sp, self.Lk.data_ptr() + self.Lk.element_size() * cl.loff, self.xdt,
+                   self.pscale.data_ptr() + 4 * cl.psoff[0] if self.f8 else None, st)
         self.g_z.run(stream)
         self.g_svd.run(stream)
         self.g_gm.run(stream)

@@ -46,8 +46,7 @@ def pick_shape(M: int, N: int) -> int:
 def make_desc(A: int, B: int, C: int, M: int, N: int, K: int, sAm: int, sAk: int, sBk: int, sBn: int,
               sCm: int, sCn: int, batch: int = 1, sAb: int = 0, sBb: int = 0, sCb: int = 0,
               alpha: float = 1.0, beta: float = 0.0, diag: float = 0.0, flags: int = 0, ksplit: int = 1,
-              skip: int = 0, scale_a: int = 0, scale_b: int = 0, sCk: int = 0, sSa: int = 0,
-              sSb: int = 0) -> L.GemmDesc:
+              skip: int = 0, scale_a: int = 0, scale_b: int = 0, sCk: int = 0) -> L.GemmDesc:
     d = L.GemmDesc()
     d.A, d.B, d.C = A, B, C
     d.M, d.N, d.K = M, N, K
@@ -57,7 +56,6 @@ def make_desc(A: int, B: int, C: int, M: int, N: int, K: int, sAm: int, sAk: int
     d.flags = flags | ((ksplit if ksplit > 1 else 0) << 8)
     d.alpha, d.beta, d.diag = alpha, beta, diag
     d.scale_a, d.scale_b, d.skip = scale_a or None, scale_b or None, skip or None
-    d.sSa, d.sSb = sSa, sSb
     return d
 
 
@@ -171,7 +169,7 @@ class StreamGemmGroup:
                 assert p % 16 == 0, "%s pointer not 16-byte aligned" % nm
             assert d.beta == 0.0 and d.diag == 0.0 and (d.flags & ~L.GEMM_OUT_BF16) == 0, \
                 "streaming GEMM: flags 0 or OUT_BF16, no beta/diag"
-            assert fp8 or not (d.scale_a or d.scale_b), "row scales only on the fp8 streaming GEMM"
+            assert not (d.scale_a or d.scale_b), "the streaming GEMM takes no scales (applied outside it)"
         self.bm = bm if bm is not None else stream_bm(max(d.M for d in descs))
         self.bn = bn
         assert (bn == 128 and self.bm in STREAM_BM) or (bn, self.bm) == (256, 128), "tile %dx%d" % (self.bm, bn)

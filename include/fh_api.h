@@ -75,7 +75,6 @@ typedef struct FhGemmDesc {
   const int* skip;        /* nullable device flag: when *skip != 0 the descriptor is a no-op */
   int64_t tile_begin;     /* written by fh_gemm_prepare */
   int32_t tiles_m, tiles_n;
-  int64_t sSa, sSb;       /* fp8 streaming GEMM: batch strides of the per-row scales scale_a / scale_b */
 } FhGemmDesc;
 
 enum {
@@ -108,12 +107,11 @@ int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhGemmDesc* de
 int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                           int grid, fh_stream_t stream);
 /* The same streaming GEMM with FP8 (OCP e4m3) operands: MFMA 16x16x128 f8f6f4, fp32 accumulate,
- * k-steps of 128 elements (one 128-byte row chunk, as for bf16).  C[b][m][n] = alpha *
- * scale_a[b * sSa + m] * scale_b[b * sSb + n] * sum_k A[b][m][k] B[b][n][k], scale_a / scale_b
- * (fp32, per row) nullable.  Contract as above with elements of one byte: leading dims and batch
- * strides multiples of 16, rows zero from K to the next multiple of 16.  The MTTKRP contractions
- * Y_c = X_c C_c (parafac2_intergrative.py:374-383) and T_hat_i = P_i^T X_i (:422-430) of the FP8
- * path. */
+ * k-steps of 128 elements (one 128-byte row chunk, as for bf16).  C = alpha A B^T; the operands'
+ * scales are applied by the caller outside the GEMM (scale_a / scale_b must be NULL).  Contract as
+ * above with elements of one byte: leading dims and batch strides multiples of 16, rows zero from
+ * K to the next multiple of 16.  The MTTKRP contractions Y_c = X_c C_c
+ * (parafac2_intergrative.py:374-383) and T_hat_i = P_i^T X_i (:422-430) of the FP8 path. */
 int fh_gemm_stream8_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                            int grid, fh_stream_t stream);
 
@@ -250,12 +248,21 @@ int fh_fill(void* p, int esize, int64_t n, uint64_t pattern, fh_stream_t stream)
 /* dst[r*dld + c] = alpha * src[r*srs + c*scs] with dtype conversion (fp32/bf16/fp64) */
 int fh_copy2d(const void* src, int sdt, int64_t srs, int64_t scs, void* dst, int ddt, int64_t dld, int64_t rows,
               int64_t cols, double alpha, fh_stream_t stream);
-/* temp rows scaled by A rows: the diag(A_i) of lhs, parafac2_intergrative.py:374 */
-typedef struct FhRowScale { float* Y; const float* A; int64_t nrows, w, s, ldy, lda; } FhRowScale;
+/* temp rows scaled by A rows: the diag(A_i) of lhs, parafac2_intergrative.py:374.
+ * Y[j][r] *= A[j / w][r] * rowf[j] * colf[r] for r < s; A, rowf, colf nullable (factor 1); the
+ * FP8 path applies the e4m3 row / column scales of Y = X C here. */
+typedef struct FhRowScale {
+  float* Y;
+  const float* A;
+  int64_t nrows, w, s, ldy, lda;
+  const float* rowf;
+  const float* colf;
+} FhRowScale;
 int fh_scale_rows(const FhRowScale* tab_dev, int n_tiles, int64_t max_elems, fh_stream_t stream);
-/* Khatri-Rao rows L[i*s + j, :] = A[i, :] * B[j, :]: lhs2 of parafac2_intergrative.py:422-423 */
+/* Khatri-Rao rows L[r][i*sp + j] = A[i][r] B[j][r] (times colf[i*sp + j] when colf is not NULL: the
+ * FP8 path's per-row scales of T_hat): lhs2 of parafac2_intergrative.py:422-423 */
 int fh_krp(const float* A, int64_t lda, const float* B, int64_t ldb, int64_t n, int64_t s, int64_t sp, void* L,
-           int l_dtype, fh_stream_t stream);
+           int l_dtype, const float* colf, fh_stream_t stream);
 /* per-block partial sums of squares: ||X||^2 of parafac2_intergrative.py:368-369 */
 int fh_sumsq_partial(const void* X, int dt, int64_t rows, int64_t cols, int64_t ld, int64_t rows_per_block,
                      double* part, fh_stream_t stream);
