@@ -418,6 +418,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restri
   const int ks = (int)(local % ksplit);
   local /= ksplit;
   const int64_t tm = local / d.tiles_n, tn = local % d.tiles_n;
+  const bool sym = (d.flags & FH_GEMM_SYMMETRIC) != 0;
+  if (sym && tn < tm) return;              // its mirror tile writes it
   const int64_t m0 = tm * BM, n0 = tn * BN;
   const int64_t kchunk = ((d.K + ksplit - 1) / ksplit + F64_BK - 1) / F64_BK * F64_BK;
   const int64_t kbeg = ks * kchunk;
@@ -497,6 +499,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restri
             if (d.beta != 0.) v += d.beta * (double)*cp;
             if (row == col && ks == 0) v += d.diag;
             *cp = (TC)v;
+            // mirror of an off-diagonal tile (a diagonal tile holds both halves itself)
+            if (sym && tn > tm) C[col * d.sCm + row * d.sCn] = (TC)v;
           }
         }
       }
@@ -586,7 +590,11 @@ extern "C" int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhG
                                 hipStream_t stream) {
   if (total_tiles <= 0) return FH_OK;
   FhGemmDesc s1{};
-  if (single) s1 = *single;
+  if (single) {
+    s1 = *single;
+    FH_REQUIRE(!(s1.flags & FH_GEMM_SYMMETRIC) || (s1.M == s1.N && s1.beta == 0. && !(s1.flags & FH_GEMM_ATOMIC)),
+               "gemm64: FH_GEMM_SYMMETRIC needs M == N, beta 0 and no atomics");
+  }
   FH_REQUIRE(descs_dev != nullptr || single != nullptr, "gemm64: no descriptor");
   dim3 grid((unsigned)total_tiles), block(NT);
   int key = (a_dtype == FH_F64) * 4 + (b_dtype == FH_F64) * 2 + (c_dtype == FH_F64);

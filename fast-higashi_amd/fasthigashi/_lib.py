@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_native", "libfh_hip.so")
 
 F32, BF16, F64, FP8 = 0, 1, 2, 3
-GEMM_ATOMIC, GEMM_OUT_BF16, GEMM_SPLIT_PARTIAL = 1, 2, 4
+GEMM_ATOMIC, GEMM_OUT_BF16, GEMM_SPLIT_PARTIAL, GEMM_SYMMETRIC = 1, 2, 4, 8
 
 
 class GemmDesc(Structure):

@@ -164,9 +164,10 @@ class CpAls:
                 q.n, q.s, q.sp, q.R, q.Dld, q.mdsplit = n, s, sp, R, Dld, self.mdsplit[c]
                 tabs[k].append(q)
             w1.append(make_desc(Tp, Dp, f4(self.W1[c]), n * sp, s, R, R, 1, Dld, 1, sp, 1))
-            gA.append(make_desc(Ap, Ap, gp[0], s, s, n, 1, sp, sp, 1, sp, 1))
-            gB.append(make_desc(Bp, Bp, gp[1], s, s, s, 1, sp, sp, 1, sp, 1))
-            gD.append(make_desc(Dp, Dp, gp[2], s, s, R, 1, Dld, Dld, 1, sp, 1))
+            sym = L.GEMM_SYMMETRIC
+            gA.append(make_desc(Ap, Ap, gp[0], s, s, n, 1, sp, sp, 1, sp, 1, flags=sym))
+            gB.append(make_desc(Bp, Bp, gp[1], s, s, s, 1, sp, sp, 1, sp, 1, flags=sym))
+            gD.append(make_desc(Dp, Dp, gp[2], s, s, R, 1, Dld, Dld, 1, sp, 1, flags=sym))
             MF, Hi = f4(self.MF[c]), self.Hinv[c].data_ptr()
             uA.append(make_desc(MF, Hi, Ap, n, s, s, sp, 1, 1, sp, sp, 1, skip=skip))
             uB.append(make_desc(MF, Hi, Bp, s, s, s, sp, 1, 1, sp, sp, 1, skip=skip))
@@ -522,7 +523,7 @@ class SweepEngine:
                 a = f4(self.temp, cl.yoff + cl.trow[t] * cl.sp)
                 d5.append(make_desc(a, a, G.data_ptr() + 8 * tl.start * G.stride(0), cl.s, cl.s, tl.width,
                                     1, cl.sp, cl.sp, 1, G.stride(1), 1, tl.rows, tl.width * cl.sp,
-                                    tl.width * cl.sp, G.stride(0)))
+                                    tl.width * cl.sp, G.stride(0), flags=L.GEMM_SYMMETRIC))
         self.g_gram = ops.Gemm64Group(d5, F32, F32, F64, dev)
         # (7) P_i = temp_i G_i^{-1/2}  (the fused solve leaves G^{-1/2} in buffer 0)
         self.pb.result = 0
@@ -613,7 +614,8 @@ class SweepEngine:
         # (12) G_M = SVD_term SVD_term^T in k-split partials (FP64)
         Gm = self.pbm.G(0)
         d12 = make_desc(self.SVDt.data_ptr(), self.SVDt.data_ptr(), self.GMpart.data_ptr(), R, R, Np,
-                        1, R, R, 1, R, 1, flags=L.GEMM_SPLIT_PARTIAL, ksplit=self.meta_ksplit, sCk=R * R)
+                        1, R, R, 1, R, 1, flags=L.GEMM_SPLIT_PARTIAL | L.GEMM_SYMMETRIC, ksplit=self.meta_ksplit,
+                        sCk=R * R)
         self.g_gm = ops.Gemm64Group([d12], F32, F32, F64, dev)
         self.Gm = Gm
         # (13) M = SVD_term^T G_M^{-1/2}

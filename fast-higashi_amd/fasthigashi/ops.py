@@ -214,6 +214,10 @@ class Gemm64Group:
     """Grouped launch of the FP64 MFMA GEMM (operands fp32 or fp64, output fp32 or fp64)."""
 
     def __init__(self, descs: Sequence[L.GemmDesc], a_dt: int, b_dt: int, c_dt: int, device="cuda"):
+        for d in descs:
+            if d.flags & L.GEMM_SYMMETRIC:
+                assert d.M == d.N and d.beta == 0.0 and not (d.flags & L.GEMM_ATOMIC), \
+                    "symmetric GEMM: M == N, beta 0, no atomics"
         arr = (L.GemmDesc * len(descs))(*descs)
         self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), 64, 64)
         self.n = len(descs)

@@ -81,6 +81,9 @@ enum {
   FH_GEMM_ATOMIC = 1,        /* C += alpha*AB with atomics (beta ignored) */
   FH_GEMM_OUT_BF16 = 2,      /* C stored as bf16 */
   FH_GEMM_SPLIT_PARTIAL = 4, /* k-split ks writes C + ks*sCk (fixed-order reduction later) */
+  FH_GEMM_SYMMETRIC = 8,     /* fp64 GEMM only: A B symmetric (a Gram); tiles below the diagonal are
+                                skipped and the tiles above it also write their mirror image; beta 0,
+                                no atomics */
 };
 
 /* host: fill tile_begin/tiles_m/tiles_n of n descriptors for a bm x bn tile; returns total tiles */

@@ -445,3 +445,26 @@ def test_quantisation_kernels_match_torch_e4m3(cuda):
     got2 = dst2.cpu()
     assert torch.equal(got2[:, :cols], ref2), int((got2[:, :cols] != ref2).sum())
     assert bool((got2[:, cols:] == 0).all())
+
+
+def test_gemm64_symmetric_flag_matches_full(cuda):
+    """FH_GEMM_SYMMETRIC (tiles below the diagonal skipped, mirrored from above) gives bitwise
+    the full product's Gram, batched and with a k-split, for s below, at and above one tile."""
+    from fasthigashi import ops, _lib as L
+    gen = torch.Generator().manual_seed(41)
+    for s, K, batch, ks in ((40, 300, 3, 1), (64, 129, 2, 1), (150, 450, 2, 1), (256, 2000, 1, 4)):
+        sp = -(-s // 8) * 8
+        A = torch.randn(batch, K, sp, generator=gen).to(cuda)
+        outs = []
+        for flags in (0, L.GEMM_SYMMETRIC):
+            C = torch.full((ks, batch, s, sp), 3.0, dtype=torch.float64, device=cuda)
+            d = ops.make_desc(A.data_ptr(), A.data_ptr(), C.data_ptr(), s, s, K, 1, sp, sp, 1, sp, 1, batch, K * sp,
+                              K * sp, s * sp, flags=flags | (L.GEMM_SPLIT_PARTIAL if ks > 1 else 0), ksplit=ks,
+                              sCk=batch * s * sp)
+            ops.Gemm64Group([d], L.F32, L.F32, L.F64, cuda).run()
+            torch.cuda.synchronize()
+            outs.append(C.cpu())
+        assert torch.equal(outs[0], outs[1]), (s, K, batch, ks)
+        ref = (A.double().transpose(1, 2) @ A.double()).cpu()[:, :s, :s]
+        got = outs[1].sum(0)[:, :, :s]
+        assert (got - ref).abs().max() < 1e-9 * ref.abs().max(), s
