@@ -78,6 +78,11 @@ class QuantRows(Structure):
                 ("sdt", c_int32), ("pad", c_int32), ("row_begin", c_int64)]
 
 
+class LossRows(Structure):
+    _fields_ = [("Q", c_void_p), ("Ct", c_void_p), ("X", c_void_p), ("W", c_int64), ("N", c_int64),
+                ("s", c_int64), ("ldq", c_int64), ("ldc", c_int64), ("ldx", c_int64), ("blk_begin", c_int64)]
+
+
 class LossChrom(Structure):
     _fields_ = [("H", c_void_p), ("A", c_void_p), ("B", c_void_p), ("gA", c_void_p), ("gBpart", c_void_p),
                 ("n", c_int64), ("s", c_int64), ("sp", c_int64), ("chunk_stride", c_int64), ("blk_begin", c_int64),
@@ -150,6 +155,8 @@ _SIGS = {
     "fh_quant_rows": (c_int, [c_void_p, c_int, c_int64, c_void_p]),
     "fh_loss_fwd": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
     "fh_loss_prepare": (c_int64, [c_void_p, c_int]),
+    "fh_loss_rows_prepare": (c_int64, [c_void_p, c_int]),
+    "fh_loss_fwd_rows": (c_int, [c_void_p, c_int, c_int64, c_int, c_void_p, c_void_p]),
     "fh_loss_bwd": (c_int, [c_void_p, c_int, c_int64, c_float, c_void_p]),
 }
 

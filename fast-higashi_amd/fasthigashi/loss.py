@@ -11,8 +11,10 @@ torch.autograd.Function so the ALS factors can be refined by gradient steps.
 Forward (no slice of the reconstruction is stored):
     C^T_c = (M D_c)^T                         fp32 MFMA GEMM
     Q_c   = P_c B_c, rows of bin i * A_i       fp32 MFMA GEMM + row scaling
-    L     = sum (X - Q C^T)^2                  fh_loss_fwd: the GEMM Q C^T with a residual
-                                               epilogue, fp64 per-tile partial sums
+    L     = sum (X - Q C^T)^2                  fh_loss_fwd_rows: a 64-row block of Q stays in LDS
+                                               while C^T slices and X stream past it, fp64
+                                               partial sums per block (or fh_loss_fwd: the GEMM
+                                               Q C^T with a residual epilogue)
 
 Backward (recomputes C^T and Q from the saved factors; gradient = MTTKRP minus factor Gram):
     F_c   = X_c C_c - Q_c (C_c^T C_c)          X read once (fp32 GEMM), Gram in fp64
@@ -48,7 +50,8 @@ class LossPlan:
 
     FWD_SHAPE = 1            # 128 x 64 tiles of the residual GEMM (measured fastest of 0, 1, 6)
 
-    def __init__(self, eng: SweepEngine, zx_target_tiles: int = 2048, max_partial_bytes: int = 4 << 30):
+    def __init__(self, eng: SweepEngine, zx_target_tiles: int = 2048, max_partial_bytes: int = 4 << 30,
+                 forward: str = "rows"):
         if eng.dtype != "fp32":
             raise ValueError("the differentiable loss needs the fp32 data of an fp32 engine")
         self.eng = eng
@@ -103,17 +106,49 @@ class LossPlan:
         self.rs_tab = ops._upload((L.RowScale * len(rs))(*rs), dev)
         self.rs_n = len(rs)
         self.rs_max = max(q.nrows * q.s for q in rs)
-        # (3) forward: residual GEMM over X, one fp64 partial per tile
-        d = [make_desc(f4(self.Q, cl.yoff), f4(self.Ct, cl.doff * Np), Xp + 4 * cl.xrow * Np, cl.W, N, cl.s,
-                       cl.sp, 1, Np, 1, Np, 1) for cl in ch]
-        bm, bn = ops.SHAPES[self.FWD_SHAPE]
-        arr = (L.GemmDesc * len(d))(*d)
-        self.fwd_total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(d), bm, bn)
-        self.fwd_n = len(d)
-        self.fwd_tab = ops._upload(arr, dev)
-        self.part = z(self.fwd_total, dt=torch.float64)
-        self.fwd_seg = torch.tensor([q.tile_begin for q in arr] + [self.fwd_total], dtype=torch.int64, device=dev)
+        # (3) forward, one fp64 partial per block.  "rows": row-stationary kernel (Q block in LDS,
+        # C^T slices and X streamed past it); "gemm": the MFMA GEMM Q C^T with a residual epilogue
+        assert forward in ("rows", "gemm")
+        self.fwd_kind = forward
         self.all_seg = torch.tensor([0, self.K], dtype=torch.int64, device=dev)
+        if forward == "gemm":
+            d = [make_desc(f4(self.Q, cl.yoff), f4(self.Ct, cl.doff * Np), Xp + 4 * cl.xrow * Np, cl.W, N, cl.s,
+                           cl.sp, 1, Np, 1, Np, 1) for cl in ch]
+            bm, bn = ops.SHAPES[self.FWD_SHAPE]
+            arr = (L.GemmDesc * len(d))(*d)
+            self.fwd_total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(d), bm, bn)
+            self.fwd_n = len(d)
+            self.fwd_tab = ops._upload(arr, dev)
+            self.part = z(self.fwd_total, dt=torch.float64)
+            self.fwd_seg = torch.tensor([q.tile_begin for q in arr] + [self.fwd_total], dtype=torch.int64,
+                                        device=dev)
+            self.fwd_order = None
+        else:
+            groups, order, bounds, base = [], [], [], 0
+            for smax in (128, 256):
+                cs = [c for c, cl in enumerate(ch) if (cl.s <= 128) == (smax == 128)]
+                if not cs:
+                    continue
+                tab = []
+                for c in cs:
+                    cl = ch[c]
+                    q = L.LossRows()
+                    q.Q, q.Ct, q.X = f4(self.Q, cl.yoff), f4(self.Ct, cl.doff * Np), Xp + 4 * cl.xrow * Np
+                    q.W, q.N, q.s, q.ldq, q.ldc, q.ldx = cl.W, N, cl.s, cl.sp, Np, Np
+                    tab.append(q)
+                arr = (L.LossRows * len(tab))(*tab)
+                tot = L.lib().fh_loss_rows_prepare(ctypes.cast(arr, ctypes.c_void_p), len(tab))
+                groups.append((smax, ops._upload(arr, dev), len(tab), tot, base))
+                for c, q in zip(cs, arr):
+                    order.append(c)
+                    bounds.append(base + q.blk_begin)
+                base += tot
+            self.fwd_groups = groups
+            self.fwd_total = base
+            self.part = z(base, dt=torch.float64)
+            self.fwd_seg = torch.tensor(bounds + [base], dtype=torch.int64, device=dev)
+            self.fwd_order = torch.tensor(order, dtype=torch.int64, device=dev)
+            self.loss_perm = z(self.K, dt=torch.float64)
         # (4) F = X C
         d = [make_desc(Xp + 4 * cl.xrow * Np, f4(self.Ct, cl.doff * Np), f4(self.F, cl.yoff), cl.W, cl.s, Np,
                        Np, 1, 1, Np, cl.sp, 1) for cl in ch]
@@ -197,10 +232,21 @@ class LossPlan:
         """Loss of the loaded factors: per chromosome into self.loss_c, the total into self.loss."""
         st = ops.stream_handle(stream)
         self._q(stream)
-        L.call("fh_loss_fwd", self.FWD_SHAPE, self.fwd_tab.data_ptr(), self.fwd_n, None, self.fwd_total,
-               self.part.data_ptr(), st)
-        L.call("fh_segment_sum", self.part.data_ptr(), self.fwd_seg.data_ptr(), self.K, self.loss_c.data_ptr(), 0, st)
-        L.call("fh_segment_sum", self.loss_c.data_ptr(), self.all_seg.data_ptr(), 1, self.loss.data_ptr(), 0, st)
+        if self.fwd_kind == "gemm":
+            L.call("fh_loss_fwd", self.FWD_SHAPE, self.fwd_tab.data_ptr(), self.fwd_n, None, self.fwd_total,
+                   self.part.data_ptr(), st)
+            L.call("fh_segment_sum", self.part.data_ptr(), self.fwd_seg.data_ptr(), self.K, self.loss_c.data_ptr(),
+                   0, st)
+            L.call("fh_segment_sum", self.loss_c.data_ptr(), self.all_seg.data_ptr(), 1, self.loss.data_ptr(), 0, st)
+            return self.loss
+        for smax, tab, n, tot, base in self.fwd_groups:
+            L.call("fh_loss_fwd_rows", tab.data_ptr(), n, tot, smax, self.part.data_ptr() + 8 * base, st)
+        # partials are grouped by tile class: per-chromosome sums in that order, then back to
+        # chromosome order
+        L.call("fh_segment_sum", self.part.data_ptr(), self.fwd_seg.data_ptr(), self.K, self.loss_perm.data_ptr(),
+               0, st)
+        L.call("fh_segment_sum", self.loss_perm.data_ptr(), self.all_seg.data_ptr(), 1, self.loss.data_ptr(), 0, st)
+        self.loss_c.index_copy_(0, self.fwd_order, self.loss_perm)
         return self.loss
 
     def backward(self, stream=None):

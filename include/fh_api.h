@@ -334,6 +334,21 @@ int fh_quant_rows(const FhQuantRows* tab_dev, int n_tab, int64_t total_rows, fh_
  * fp64 for every tile t < total_tiles; flags 0, no k-split; shape 0, 1 or 6. */
 int fh_loss_fwd(int shape, const FhGemmDesc* descs_dev, int n_desc, const FhGemmDesc* single_host,
                 int64_t total_tiles, double* part, fh_stream_t stream);
+/* The same forward, row-stationary: per chromosome Q (W x ldq, k = s contiguous, columns from s
+ * to the next multiple of 4 zero), C^T (s rows of ldc cells; rows from s to the next multiple of 4
+ * zero) and X (W x ldx); ldc, ldx >= N rounded up to 32, multiples of 4, 16-byte aligned bases;
+ * s <= smax (128 or 256).  One partial per block of 64 rows of a chromosome (blk_begin filled by
+ * fh_loss_rows_prepare, which returns the block count).  No counterpart. */
+typedef struct FhLossRows {
+  const float* Q;
+  const float* Ct;
+  const float* X;
+  int64_t W, N, s, ldq, ldc, ldx;
+  int64_t blk_begin;
+} FhLossRows;
+int64_t fh_loss_rows_prepare(FhLossRows* tab_host, int n);
+int fh_loss_fwd_rows(const FhLossRows* tab_dev, int n_tab, int64_t total_blocks, int smax, double* part,
+                     fh_stream_t stream);
 /* Backward for A and B from H_i = P_i^T (X_i C - Q_i C^T C) (n x s x sp per chromosome):
  * gA[i, r] = scale sum_p B[p, r] H_i[p, r] (all r < sp; 0 for r >= s) and, per chunk k of
  * FH_LOSS_CHUNK bins, gBpart[k * chunk_stride + p * sp + r] = scale sum_{i in chunk k} H_i[p, r] A[i, r]

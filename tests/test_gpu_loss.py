@@ -59,6 +59,27 @@ def small():
     return data, eng
 
 
+@pytest.mark.parametrize("forward", ["rows", "gemm"])
+def test_forward_kernels_match_oracle_small(cuda, small, forward):
+    """Both forward kernels (row-stationary and GEMM-epilogue) against the FP64 oracle."""
+    from oracle import ref_np as R
+    from fasthigashi.loss import LossPlan
+    data, eng = small
+    plan = LossPlan(eng, forward=forward)
+    plan.load(eng.A_all, eng.B_all, eng.D_all, eng.M)
+    got = float(plan.forward().item())
+    per = plan.loss_c.cpu().numpy()
+    F = R.Factors(*eng.get_factors())
+    proj = eng.projections()
+    ref = R.parafac2_loss(data, F, proj)
+    _say("small %s forward: %.9e oracle %.9e" % (forward, got, ref))
+    assert abs(got - ref) < 1e-6 * ref
+    assert abs(per.sum() - got) < 1e-12 * got
+    for c in range(len(data)):
+        sub = R.parafac2_loss([data[c]], R.Factors([F.A[c]], [F.B[c]], [F.D[c]], F.M), [proj[c]])
+        assert abs(per[c] - sub) < 1e-6 * sub, c
+
+
 @pytest.mark.parametrize("point", ["als", "perturbed"])
 def test_loss_and_gradient_match_oracle_small(cuda, small, point):
     """At the ALS solution the A, B, D gradients nearly vanish (the CP-ALS step just solved for
