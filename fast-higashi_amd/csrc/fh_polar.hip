@@ -31,13 +31,15 @@ __device__ __forceinline__ int find_group(const FhNsGroup* g, int n, int64_t t) 
   return lo;
 }
 
-// 64x64 output tile of  alpha * L * R  (+ diag on the diagonal),  L, R row-major s x s.
-// Writes out and, if res != null, accumulates sum((out - I)^2) over the tile.
+// 64x64 output tile of  alpha * L * R  (+ diag on the diagonal),  L, R row-major s x s, in
+// k-steps of BKD (smem: 2 * 64 * (BKD + 1) doubles).  The MFMA sequence over k does not depend
+// on BKD.  Writes out and, if res != null, accumulates sum((out - I)^2) over the tile.
+template <int BKD = 16>
 __device__ __forceinline__ void tile_gemm_f64(const double* __restrict__ L, const double* __restrict__ R,
                                               double* __restrict__ out, int64_t s, int64_t ld,
                                               int64_t m0, int64_t n0, double alpha, double diag,
                                               double* res, double* smem) {
-  constexpr int BKD = 16, LR = BKD + 1;
+  constexpr int LR = BKD + 1, PER = 64 * BKD / NT;
   double* As = smem;
   double* Bs = smem + 64 * LR;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -50,9 +52,9 @@ __device__ __forceinline__ void tile_gemm_f64(const double* __restrict__ L, cons
   for (int64_t k0 = 0; k0 < s; k0 += BKD) {
     __syncthreads();
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
+    for (int p = 0; p < PER; ++p) {
       int e = tid + p * NT;
-      int r = e >> 4, k = e & 15;
+      int r = e / BKD, k = e % BKD;
       // L[m][k] (row-major, k contiguous)
       As[r * LR + k] = (m0 + r < s && k0 + k < s) ? L[(m0 + r) * ld + k0 + k] : 0.;
       // R[k][n] (row-major, n contiguous).  The iterates are symmetric only up to
@@ -99,6 +101,10 @@ __device__ __forceinline__ void tile_gemm_f64(const double* __restrict__ L, cons
   }
 }
 
+// k-step of the per-iteration Newton-Schulz products: four k-steps of 64 at s = 256 (one
+// global-load round trip each) instead of sixteen of 16
+constexpr int kNsBK = 64;
+
 __global__ __launch_bounds__(NT) void ns_init_kernel(const FhNsGroup* __restrict__ groups, int n_groups,
                                                      const int64_t* __restrict__ mat_begin, int n_iter) {
   // one workgroup per matrix; blockIdx.x = global matrix index
@@ -136,10 +142,10 @@ __global__ __launch_bounds__(NT) void ns_init_kernel(const FhNsGroup* __restrict
   for (int it = threadIdx.x; it < n_iter; it += NT) g.res[(int64_t)it * g.count + b] = 0.;
 }
 
-// T = 1.5 I - 0.5 Z_k Y_k  for matrices still iterating.
+// T = diag I + alpha Z_k Y_k for matrices still iterating (1.5 I - 0.5 Z_k Y_k unscaled).
 __global__ __launch_bounds__(NT) void ns_t_kernel(const FhNsGroup* __restrict__ groups, int n_groups,
-                                                  int iter, double tol) {
-  __shared__ double smem[2 * 64 * 17];
+                                                  int iter, double tol, double alpha, double diag) {
+  __shared__ double smem[2 * 64 * (kNsBK + 1)];
   const int64_t t = blockIdx.x;
   const FhNsGroup g = groups[find_group(groups, n_groups, t)];
   int64_t local = t - g.tile_begin;
@@ -159,13 +165,13 @@ __global__ __launch_bounds__(NT) void ns_t_kernel(const FhNsGroup* __restrict__ 
   const double* Y = (cur ? g.Y1 : g.Y0) + b * g.stride;
   const double* Z = (cur ? g.Z1 : g.Z0) + b * g.stride;
   double* T = g.T + b * g.stride;
-  tile_gemm_f64(Z, Y, T, g.s, g.ld, tm * 64, tn * 64, -0.5, 1.5, res_now, smem);
+  tile_gemm_f64<kNsBK>(Z, Y, T, g.s, g.ld, tm * 64, tn * 64, alpha, diag, res_now, smem);
 }
 
 // Y_{k+1} = Y_k T, Z_{k+1} = T Z_k  (grid: 2 x tiles); converged matrices are copied.
 __global__ __launch_bounds__(NT) void ns_yz_kernel(const FhNsGroup* __restrict__ groups, int n_groups,
                                                    int iter, double tol) {
-  __shared__ double smem[2 * 64 * 17];
+  __shared__ double smem[2 * 64 * (kNsBK + 1)];
   const int64_t t = blockIdx.x >> 1;
   const int which = blockIdx.x & 1;
   const FhNsGroup g = groups[find_group(groups, n_groups, t)];
@@ -191,8 +197,8 @@ __global__ __launch_bounds__(NT) void ns_yz_kernel(const FhNsGroup* __restrict__
     return;
   }
   const double* T = g.T + b * g.stride;
-  if (which == 0) tile_gemm_f64(Yc, T, Yn, g.s, g.ld, m0, n0, 1.0, 0.0, nullptr, smem);
-  else tile_gemm_f64(T, Zc, Zn, g.s, g.ld, m0, n0, 1.0, 0.0, nullptr, smem);
+  if (which == 0) tile_gemm_f64<kNsBK>(Yc, T, Yn, g.s, g.ld, m0, n0, 1.0, 0.0, nullptr, smem);
+  else tile_gemm_f64<kNsBK>(T, Zc, Zn, g.s, g.ld, m0, n0, 1.0, 0.0, nullptr, smem);
 }
 
 // After the last iteration (result in buffer `fin`): Zf = Z / sqrt(c) (= G^{-1/2}),
@@ -609,6 +615,7 @@ struct GjBuf {
 // (row slot kr, column slot kq = kr RG / CG) are compile-time: publishing row / column k and
 // rewriting them after the rank-1 update a <- a - u v^T (u = column k, v = row k / p) are
 // plain masked register moves, with no selects over slots.  The pivot owner broadcasts 1 / p.
+// The row is scaled by -1 / p, so the update is fma(u, -v, a) (v_fmac, no sign flips of u).
 template <int RG, int CG, int SMAX>
 __device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int s, double* rowb, double* colb,
                                           double* ipb) {
@@ -641,13 +648,13 @@ __device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int
         }
       }
       __syncthreads();
-      const double ip = ipb[bf];
+      const double ip = ipb[bf], nip = -ip;
       double v[CPTP], u[RPTP];
 #pragma unroll
       for (int q = 0; q < CPTP; q += 2) {
         const doublex2 t = *(const doublex2*)(rb + tx * CPTP + q);
-        v[q] = t[0] * ip;
-        v[q + 1] = t[1] * ip;
+        v[q] = t[0] * nip;
+        v[q + 1] = t[1] * nip;
       }
 #pragma unroll
       for (int r = 0; r < RPTP; r += 2) {
@@ -658,15 +665,15 @@ __device__ __forceinline__ void gj_invert(double (&a)[SMAX / RG][SMAX / CG], int
 #pragma unroll
       for (int r = 0; r < RPT; ++r)
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) a[r][q] = fma(-u[r], v[q], a[r][q]);
+        for (int q = 0; q < CPT; ++q) a[r][q] = fma(u[r], v[q], a[r][q]);
       // row k <- A[k, :] / p, column k <- -A[:, k] / p, (k, k) <- 1 / p
       if (ty == ky) {
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) a[kr][q] = v[q];
+        for (int q = 0; q < CPT; ++q) a[kr][q] = -v[q];
       }
       if (tx == kx) {
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) a[r][kq] = -u[r] * ip;
+        for (int r = 0; r < RPT; ++r) a[r][kq] = u[r] * nip;
         if (ty == ky) a[kr][kq] = ip;
       }
     }
@@ -868,11 +875,11 @@ extern "C" int fh_polar_newton(const FhNsGroup* groups_dev, int n_groups, const 
   hipLaunchKernelGGL(newton_init_kernel, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
                      iters_dev);
   for (int step = 0; step < max_iter; ++step) {
-    hipLaunchKernelGGL(newton_gj_kernel<5>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev,
+    hipLaunchKernelGGL(newton_gj_kernel<5>, grid, dim3(GjCfg<5>::RG * GjCfg<5>::CG), 0, stream, groups_dev, n_groups, mat_begin_dev,
                        iters_dev, step, tol);
-    hipLaunchKernelGGL(newton_gj_kernel<4>, grid, dim3(512), 0, stream, groups_dev, n_groups, mat_begin_dev,
+    hipLaunchKernelGGL(newton_gj_kernel<4>, grid, dim3(GjCfg<4>::RG * GjCfg<4>::CG), 0, stream, groups_dev, n_groups, mat_begin_dev,
                        iters_dev, step, tol);
-    hipLaunchKernelGGL(newton_gj_kernel<2>, grid, dim3(256), 0, stream, groups_dev, n_groups, mat_begin_dev,
+    hipLaunchKernelGGL(newton_gj_kernel<2>, grid, dim3(GjCfg<2>::RG * GjCfg<2>::CG), 0, stream, groups_dev, n_groups, mat_begin_dev,
                        iters_dev, step, tol);
     if (step > 0) {
       hipLaunchKernelGGL(newton_kw_kernel<5>, grid, dim3(640), 0, stream, groups_dev, n_groups, mat_begin_dev,
@@ -939,8 +946,23 @@ extern "C" int fh_ns_run(const FhNsGroup* groups_dev, int n_groups, const int64_
   hipLaunchKernelGGL(ns_init_kernel, dim3((unsigned)total_mats), dim3(NT), 0, stream, groups_dev, n_groups,
                      mat_begin_dev, n_iter);
   FH_CHECK_LAUNCH();
+  // Scaled iteration (Chen & Chow, "A stable scaling of Newton-Schulz for improving the sign
+  // function computation of a Hermitian matrix", 2014): in sign-function form the step is
+  // x <- a x (3 - a^2 x^2) / 2 on the singular values x of Y_0^{1/2} (in (0, 1] after the
+  // Frobenius scaling), with a = sqrt(3 / (1 + l + l^2)) for a lower bound l of the interval;
+  // the map keeps (0, 1] in (0, 1], lifts small values by up to 1.5 sqrt(3) per step instead
+  // of 1.5, and l follows the same map.  l_0 = 1e-4 (condition number 1e8 of G); smaller
+  // singular values still converge, as in the unscaled iteration once a reaches 1.  The
+  // coupled form: T = (a / 2) (3 I - a^2 Z Y).  The limit (G / c)^{+-1/2} does not change.
+  double ell = 1e-4;
   for (int it = 0; it < n_iter; ++it) {
-    hipLaunchKernelGGL(ns_t_kernel, dim3((unsigned)total_tiles), dim3(NT), 0, stream, groups_dev, n_groups, it, tol);
+    double a = 1.;
+    if (ell < 1. - 1e-3) {
+      a = sqrt(3. / (1. + ell + ell * ell));
+      ell = a * ell * (3. - a * a * ell * ell) / 2.;
+    }
+    hipLaunchKernelGGL(ns_t_kernel, dim3((unsigned)total_tiles), dim3(NT), 0, stream, groups_dev, n_groups, it, tol,
+                       -0.5 * a * a * a, 1.5 * a);
     FH_CHECK_LAUNCH();
     hipLaunchKernelGGL(ns_yz_kernel, dim3((unsigned)(2 * total_tiles)), dim3(NT), 0, stream, groups_dev, n_groups, it, tol);
     FH_CHECK_LAUNCH();
@@ -961,9 +983,9 @@ extern "C" int fh_spd_inverse(const FhSpdSys* sys_dev, int n, int64_t max_s, int
   size_t lds = (size_t)(max_s * (max_s + 1) / 2) * sizeof(double);
   FH_REQUIRE(lds <= (size_t)CHOL_LDS, "spd_inverse: s=%lld exceeds LDS", (long long)max_s);
   const dim3 grid(n);
-  if (max_s > 128) hipLaunchKernelGGL(spd_gj_kernel<5>, grid, dim3(512), 0, stream, sys_dev);
-  if (max_s > 64) hipLaunchKernelGGL(spd_gj_kernel<4>, grid, dim3(512), 0, stream, sys_dev);
-  hipLaunchKernelGGL(spd_gj_kernel<2>, grid, dim3(256), 0, stream, sys_dev);
+  if (max_s > 128) hipLaunchKernelGGL(spd_gj_kernel<5>, grid, dim3(GjCfg<5>::RG * GjCfg<5>::CG), 0, stream, sys_dev);
+  if (max_s > 64) hipLaunchKernelGGL(spd_gj_kernel<4>, grid, dim3(GjCfg<4>::RG * GjCfg<4>::CG), 0, stream, sys_dev);
+  hipLaunchKernelGGL(spd_gj_kernel<2>, grid, dim3(GjCfg<2>::RG * GjCfg<2>::CG), 0, stream, sys_dev);
   if (max_s > 160) hipLaunchKernelGGL(spd_inverse_kernel, grid, dim3(NT), lds, stream, sys_dev, status_dev);
   FH_CHECK_LAUNCH();
   return FH_OK;
