@@ -28,7 +28,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-WHAT = {"y": "Y = X C", "that": "T_hat = P^T X", "ttilde": "T_tilde = T_hat M"}
+WHAT = {"y": "Y = X C", "that": "T_hat = P^T X", "ttilde": "T_tilde = T_hat M", "z": "Z = T_hat^T L^T"}
 
 
 def launch_work(eng, phase):
@@ -63,6 +63,9 @@ def kernel_name(phase, head):
     tag = phase.split(":")[1].split("#")[0]
     if tag.startswith("s"):
         bm, bn = tag[1:].split("x")
+        if phase.startswith("z"):
+            return ("gemm_stream_kernel, bf16 operands, %sx%s tiles, global->LDS ring, T_hat read cells-"
+                    "contiguous with ds_read_b64_tr_b16" % (bm, bn))
         opnd = "fp8 (e4m3)" if head == "fp8" and not phase.startswith("ttilde") else "bf16"
         return "gemm_stream_kernel, %s operands, %sx%s tiles, global->LDS ring" % (opnd, bm, bn)
     from fasthigashi import ops

@@ -464,7 +464,9 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(const FhGemmDesc* __restri
       la.load(ra, tid, k0 + F64_BK, kend);
       lb.load(rb, tid, k0 + F64_BK, kend);
     }
-#pragma unroll
+    // not unrolled: unrolling hoists every LDS fragment load of the k-tile (64 more VGPRs,
+    // two waves per SIMD instead of three)
+#pragma unroll 1
     for (int kk = 0; kk < F64_BK; kk += 4) {
       double a[2], bb[2];
 #pragma unroll

@@ -20,6 +20,16 @@
 // run time, which keeps the kernel small enough that its once-per-tile output path stays in
 // the instruction cache.
 //
+// A-m-contiguous mode (AMN, bf16, 128 x 160 tiles): A is stored k-major with m contiguous
+// (element (m, k) at m + k sAk; Z = T_hat^T L^T reads T_hat so).  A stage then holds A as 64
+// k-rows of BM * 2 = 256 bytes; a 1 KB load fills 4 of them (lane l: position l % 16 of row
+// 4 q + l / 16), and chunk c of row r sits at position c ^ amn_swz(r).  The MFMA operand of
+// a 16-row m block, 8 consecutive k per lane, comes from two ds_read_b64_tr_b16: in each
+// group of 16 lanes, lane j addresses 4 consecutive m of k-row j / 4 (m offset 4 (j % 4)), and
+// lane i receives element i % 4 of the rows of lanes 4 q + i / 4, q = 0..3, i.e. the four
+// k-rows at m offset i.  amn_swz spreads the 16 k-rows one such read touches over all 16
+// chunk positions twice (two LDS passes, the minimum for 512 bytes).
+//
 // Contract (checked by the host): both operands k-contiguous, leading dims and batch strides
 // multiples of 8 elements, 16-byte aligned bases, and either K % 8 == 0 or every row zero
 // from K up to the next multiple of 8 (a chunk that K ends inside is read whole).  flags: 0
@@ -128,13 +138,24 @@ struct LoadRef {
   int nk;            // k-steps
   int nv_last;       // 16-byte chunks of the last k-step inside K (8: all)
   bool live;
+  // AMN: bytes per k-step of A (64 k-rows), A k-rows of the last step inside K, and the load
+  // instructions whose lane chunk lies past M (they load the zero block at every k-step)
+  uint64_t kstepA;
+  int krem;
+  uint32_t zmask;
 };
 
-template <class C, bool F8>
+// AMN chunk swizzle of A k-row r (16 chunk positions of 16 bytes)
+__device__ __forceinline__ int amn_swz(int r) { return 2 * ((r & 3) | ((r >> 2) & 4)); }
+
+template <class C, bool F8, bool AMN>
 __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int& di, int64_t t, int64_t total,
                                                   int wave, int lane) {
   constexpr int ES = F8 ? 1 : 2, EK = 128 / ES;    // bytes per element, elements per k-step
   LoadRef<C::D> r;
+  r.kstepA = 0;
+  r.krem = EK;
+  r.zmask = 0;
   if (t >= total) {
 #pragma unroll
     for (int q = 0; q < C::D; ++q) r.p[q] = (const uint8_t*)g_zero16;
@@ -142,6 +163,7 @@ __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int&
     r.nk = 1;
     r.nv_last = 8;
     r.live = false;
+    r.zmask = ~0u;
     return r;
   }
   di = advance_desc(descs, n_desc, di, t);
@@ -155,6 +177,14 @@ __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int&
     const int i = wave + C::NW * q;                 // load instruction index in the stage
     const bool isA = i < C::DA;                     // uniform
     const int blk = isA ? i : i - C::DA;
+    if (AMN && isA) {
+      // k-row 4 blk + l / 16 of the step, chunk (l % 16) ^ amn_swz(row): 8 m from m0 + 8 c
+      const int row = blk * 4 + (lane >> 4);
+      const int c = (lane & 15) ^ amn_swz(row);
+      if (pos.m0 + 8 * c >= d.M) r.zmask |= 1u << q;
+      r.p[q] = baseA + ES * ((int64_t)row * d.sAk + 8 * c);
+      continue;
+    }
     const int row = blk * 8 + (lane >> 3);
     const int c = (lane & 7) ^ (row & 7);
     const int64_t rr = min((int64_t)row, (isA ? rows_a : rows_b) - 1);
@@ -164,6 +194,10 @@ __device__ __forceinline__ LoadRef<C::D> load_ref(CDesc* descs, int n_desc, int&
   r.nk = (int)((d.K + EK - 1) / EK);
   r.nv_last = (int)(((d.K - EK * (int64_t)(r.nk - 1)) * ES + 15) / 16);
   r.live = true;
+  if (AMN) {
+    r.kstepA = (uint64_t)EK * ES * d.sAk;
+    r.krem = (int)(d.K - EK * (int64_t)(r.nk - 1));
+  }
   return r;
 }
 
@@ -179,8 +213,28 @@ __device__ __forceinline__ void dma16(const void* src, uint32_t lds) {
 // issue the loads of k-step kk of tile r into the stage at LDS address st.  A lane's chunk
 // (l % 8) ^ (l / 8) is the same for all its loads; on the last k-step, lanes whose chunk lies
 // past K load the zero block, so the padding of K up to a multiple of 64 is never fetched.
-template <class C>
+template <class C, bool AMN>
 __device__ __forceinline__ void issue_stage(const LoadRef<C::D>& r, int kk, uint32_t st, int wave, int lane) {
+  if constexpr (AMN) {
+    // A: k-rows past K on the last step and chunks past M load the zero block
+    const bool last = kk == r.nk - 1;              // uniform
+    const uint64_t ka = (uint64_t)kk * r.kstepA;
+    const uint32_t kb = (uint32_t)kk * r.kstep;
+    const bool bin = !last || ((lane & 7) ^ (lane >> 3)) < r.nv_last;
+#pragma unroll
+    for (int q = 0; q < C::D; ++q) {
+      const int i = wave + C::NW * q;
+      const void* src;
+      if (i < C::DA) {                             // uniform
+        const bool z = ((r.zmask >> q) & 1u) || (last && i * 4 + (lane >> 4) >= r.krem);
+        src = z ? (const void*)g_zero16 : (const void*)(r.p[q] + ka);
+      } else {
+        src = bin ? (const void*)(r.p[q] + kb) : (const void*)g_zero16;
+      }
+      dma16(src, st + i * 1024);
+    }
+    return;
+  }
   const uint32_t kb = (uint32_t)kk * r.kstep;
   if (kk == r.nk - 1 && r.nv_last < 8) {           // uniform
     const bool in = ((lane & 7) ^ (lane >> 3)) < r.nv_last;
@@ -201,9 +255,9 @@ __device__ __forceinline__ void wait_vm_barrier() {
   asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" : : "n"(N) : "memory");
 }
 
-template <class C, bool F8>
-__device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[C::WM_T][C::WN_T], int wm, int wn,
-                                              int lane) {
+template <class C, bool F8, bool AMN>
+__device__ __forceinline__ void compute_stage(const uint8_t* st, uint32_t st_lds, floatx4 (&acc)[C::WM_T][C::WN_T],
+                                              int wm, int wn, int lane) {
   constexpr int WM_T = C::WM_T, WN_T = C::WN_T;
   const uint8_t* As = st;
   const uint8_t* Bs = st + C::BM * 128;
@@ -247,13 +301,37 @@ __device__ __forceinline__ void compute_stage(const uint8_t* st, floatx4 (&acc)[
     }
   }
   u32x4 af[WM_T][2];
+  if constexpr (AMN) {
+    // lane l: k = 16 (l / 16) + 8 t + 0..7 (the k of the B fragment's chunk 2 (l / 16) + t)
+    // at m offset l % 16; the lane addresses k-row 16 g + 8 t + 4 h + (l % 16) / 4, m offset
+    // 4 (l % 4) of the block
+    typedef short v4s __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) v4s* lds_v4s;
+    const int g = lane >> 4, jq = (lane & 15) >> 2, b = lane & 3;
 #pragma unroll
-  for (int i = 0; i < WM_T; ++i) {
-    const int row = wm * WM_T * 16 + i * 16 + (lane & 15);
+    for (int i = 0; i < WM_T; ++i) {
+      const int c = ((wm * WM_T * 16 + i * 16) >> 3) + (b >> 1);
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int c = 2 * (lane >> 4) + t;
-      af[i][t] = *(const u32x4*)(As + row * 128 + ((c ^ (row & 7)) << 4));
+      for (int t = 0; t < 2; ++t) {
+        v4s h2[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = 16 * g + 8 * t + 4 * h + jq;
+          const uint32_t a = st_lds + r * (C::BM * 2) + ((c ^ amn_swz(r)) << 4) + (b & 1) * 8;
+          h2[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s)(uintptr_t)a);
+        }
+        af[i][t] = __builtin_bit_cast(u32x4, __builtin_shufflevector(h2[0], h2[1], 0, 1, 2, 3, 4, 5, 6, 7));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i) {
+      const int row = wm * WM_T * 16 + i * 16 + (lane & 15);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = 2 * (lane >> 4) + t;
+        af[i][t] = *(const u32x4*)(As + row * 128 + ((c ^ (row & 7)) << 4));
+      }
     }
   }
   // B is the first MFMA operand, so acc[i][j] holds C^T blocks: lane l has
@@ -338,9 +416,10 @@ __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C:
 // last tile).  The output
 // stores count in vmcnt too; the stage wait stays exact because a wave's loads retire in
 // issue order, so vmcnt <= n leaves at most its n youngest loads outstanding.
-template <class C, bool F8>
+template <class C, bool F8, bool AMN>
 __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __restrict__ descs_arg, int n_desc,
                                                             int64_t total_tiles) {
+  static_assert(!AMN || (!F8 && C::BM == 128), "A-m-contiguous mode: bf16, 128-row tiles");
   constexpr int WM_T = C::WM_T, WN_T = C::WN_T, STAGES = C::STAGES;
   constexpr int EK = F8 ? 128 : 64;
   CDesc* descs = (CDesc*)descs_arg;
@@ -355,7 +434,7 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
   // load cursor
   int64_t lt = blockIdx.x;
   int dl = 0;
-  LoadRef<C::D> lr = load_ref<C, F8>(descs, n_desc, dl, lt, total_tiles, wave, lane);
+  LoadRef<C::D> lr = load_ref<C, F8, AMN>(descs, n_desc, dl, lt, total_tiles, wave, lane);
   int lk = 0;
   // compute cursor
   int64_t ct = blockIdx.x;
@@ -370,11 +449,11 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
     for (int j = 0; j < WN_T; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   auto issue = [&](int stage) {
-    issue_stage<C>(lr, lk, ring_lds + stage * C::STAGE_BYTES, wave, lane);
+    issue_stage<C, AMN>(lr, lk, ring_lds + stage * C::STAGE_BYTES, wave, lane);
     if (++lk >= lr.nk) {
       lk = 0;
       lt += stride;
-      lr = load_ref<C, F8>(descs, n_desc, dl, lt, total_tiles, wave, lane);
+      lr = load_ref<C, F8, AMN>(descs, n_desc, dl, lt, total_tiles, wave, lane);
     }
   };
 #pragma unroll
@@ -382,10 +461,15 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
   for (int k = 0; sr.live; ++k) {
     wait_vm_barrier<C::D * (STAGES - 2)>();
     issue((k + STAGES - 1) % STAGES);
-    compute_stage<C, F8>(ring + (k % STAGES) * C::STAGE_BYTES, acc, wm, wn, lane);
+    compute_stage<C, F8, AMN>(ring + (k % STAGES) * C::STAGE_BYTES, ring_lds + (k % STAGES) * C::STAGE_BYTES, acc,
+                              wm, wn, lane);
     if (++ck >= sr.nk) {
-      if (sr.bf16) store_tile<2, C>(sr, acc, wm, wn, lane, SO[wave]);
-      else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);
+      if constexpr (AMN) {
+        store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);   // fp32 output only (host-checked)
+      } else {
+        if (sr.bf16) store_tile<2, C>(sr, acc, wm, wn, lane, SO[wave]);
+        else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);
+      }
 #pragma unroll
       for (int i = 0; i < WM_T; ++i)
 #pragma unroll
@@ -404,15 +488,15 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
 // (bm, bn): (64 | 96 | 128 | 160, 128) with 4 waves and 4 stages, or (128, 256) with 8 waves
 // and 3 stages.  grid: persistent workgroups (0 = one per CU).
 namespace {
-template <class C, bool F8>
+template <class C, bool F8, bool AMN = false>
 int launch_stream(const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL((gemm_stream_kernel<C, F8>), dim3(grid), dim3(C::NT), 0, stream, descs_dev, n_desc,
+  hipLaunchKernelGGL((gemm_stream_kernel<C, F8, AMN>), dim3(grid), dim3(C::NT), 0, stream, descs_dev, n_desc,
                      total_tiles);
   FH_CHECK_LAUNCH();
   return FH_OK;
 }
 
-template <bool F8>
+template <bool F8, bool AMN = false>
 int stream_dispatch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles, int grid,
                     hipStream_t stream) {
   if (total_tiles <= 0) return FH_OK;
@@ -426,6 +510,13 @@ int stream_dispatch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int
     grid = cus;
   }
   if ((int64_t)grid > total_tiles) grid = (int)total_tiles;
+  if constexpr (AMN) {
+    // 4 waves stacked along m (32 rows each), each over all 160 columns (fp32 output groups)
+    if (bm == 128 && bn == 160)
+      return launch_stream<StreamCfg<2, 10, 4, 1, 4>, false, true>(descs_dev, n_desc, total_tiles, grid, stream);
+    fh_set_error("gemm_stream_mn: unsupported tile %d x %d (128 x 160)", bm, bn);
+    return FH_ERR_ARG;
+  }
   if (bn == 128) {
     switch (bm) {
       case 64: return launch_stream<StreamCfg<2, 4, 2, 2, 4>, F8>(descs_dev, n_desc, total_tiles, grid, stream);
@@ -445,6 +536,11 @@ int stream_dispatch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int
 extern "C" int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                                      int grid, hipStream_t stream) {
   return stream_dispatch<false>(bm, bn, descs_dev, n_desc, total_tiles, grid, stream);
+}
+
+extern "C" int fh_gemm_stream_mn_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc,
+                                        int64_t total_tiles, int grid, hipStream_t stream) {
+  return stream_dispatch<false, true>(bm, bn, descs_dev, n_desc, total_tiles, grid, stream);
 }
 
 extern "C" int fh_gemm_stream8_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,

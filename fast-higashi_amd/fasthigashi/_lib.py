@@ -146,6 +146,7 @@ _SIGS = {
     "fh_colsum_add": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p, c_int64,
                               c_void_p]),
     "fh_gemm_stream8_launch": (c_int, [c_int, c_int, c_void_p, c_int, c_int64, c_int, c_void_p]),
+    "fh_gemm_stream_mn_launch": (c_int, [c_int, c_int, c_void_p, c_int, c_int64, c_int, c_void_p]),
     "fh_amax_rows": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
     "fh_amax_col_chunks": (c_int, [c_void_p, c_int, c_int64, c_int64, c_int64, c_int64, c_void_p, c_void_p,
                                    c_void_p]),

@@ -6,7 +6,8 @@ descriptor table launched with one tile shape.  The engine picks shapes by rule 
 covering s, one tile column over s, ...); `tune_engine` instead times every tile shape the
 kernel family has on the engine's own data and keeps the fastest, group by group:
 
-* streaming bf16 / fp8 GEMM: (bm, bn) in (64 | 96 | 128 | 160, 128) and (128, 256);
+* streaming bf16 / fp8 GEMM: (bm, bn) in (64 | 96 | 128 | 160, 128) and (128, 256) (the
+  m-contiguous-A mode of Z has the one shape 128 x 160 and is left as it is);
 * tiled fp32 / bf16 MFMA GEMM: the seven shapes 128x128, 128x64, 64x64, 128x160, 64x128,
   256x128, 128x256.
 
@@ -72,6 +73,8 @@ def _group_key(family: str, g) -> str:
 
 
 def _candidates(g, device):
+    if getattr(g, "a_mn", False):
+        return None, None                      # one tile shape (128 x 160) in this mode
     if isinstance(g, ops.StreamGemmGroup):
         return [((bm, bn), (lambda bm=bm, bn=bn: ops.StreamGemmGroup(g.descs, bm=bm, bn=bn, grid=g.grid,
                                                                       device=device, fp8=g.fp8)))
@@ -92,7 +95,7 @@ def _merge(fam, groups, device, reps, cache, report, verbose):
     for g in groups:
         if isinstance(g, (ops.StreamGemmGroup, ops.GemmGroup)):
             k = (type(g).__name__, getattr(g, "dtype", None), getattr(g, "ak", None), getattr(g, "bk", None),
-                 getattr(g, "fp8", None), _shape_of(g))
+                 getattr(g, "fp8", None), getattr(g, "a_mn", None), _shape_of(g))
             by.setdefault(k, []).append(g)
     out = [g for g in groups if not isinstance(g, (ops.StreamGemmGroup, ops.GemmGroup))]
     for k, gs in by.items():
@@ -102,7 +105,7 @@ def _merge(fam, groups, device, reps, cache, report, verbose):
         descs = [d for g in gs for d in g.descs]
         shape = _shape_of(gs[0])
         make = ((lambda: ops.StreamGemmGroup(descs, bm=shape[0], bn=shape[1], grid=gs[0].grid, device=device,
-                                             fp8=gs[0].fp8))
+                                             fp8=gs[0].fp8, a_mn=gs[0].a_mn))
                 if isinstance(gs[0], ops.StreamGemmGroup) else
                 (lambda: ops.GemmGroup(descs, gs[0].dtype, shape, device)))
         key = "%s|merge|%s" % (_group_key(fam, gs[0]), sorted((d.M, d.N, d.K, d.batch) for d in descs))

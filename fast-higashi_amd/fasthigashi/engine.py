@@ -601,12 +601,16 @@ class SweepEngine:
                                        for k, v in sorted(by.items())])
         else:
             self.g_that = MultiGemm(d8, self.xdt, lambda d: shape_for_m(d.M), dev)
-        # (10) Zt_c = T_hat_c^T L_c^T
+        # (10) Zt_c = T_hat_c^T L_c^T.  bf16: streamed, T_hat (positions x cells, cells contiguous)
+        # read as stored (m-contiguous A, transposed LDS reads), one 128 x 160 launch
         d10 = []
         for cl in self.chroms:
             d10.append(make_desc(Thp + xe * cl.toff * Np, Lp + xe * cl.loff, f4(self.Zt, cl.doff),
                                  Np, cl.s, cl.n * cl.sp, 1, Np, 1, cl.n * cl.sp, Dld, 1))
-        self.g_z = MultiGemm(d10, self.xdt, lambda d: shape_for_n(d.N), dev)
+        if lowp:
+            self.g_z = StreamMulti([ops.StreamGemmGroup(d10, bm=128, bn=160, device=dev, a_mn=True)])
+        else:
+            self.g_z = MultiGemm(d10, self.xdt, lambda d: shape_for_n(d.N), dev)
         # (11) SVD_term^T = Zt D_all^T
         d11 = make_desc(self.Zt.data_ptr(), self.D_all.data_ptr(), self.SVDt.data_ptr(), Np, R, Dld,
                         Dld, 1, 1, Dld, R, 1)
@@ -680,9 +684,13 @@ class SweepEngine:
         return out
 
     def launch_groups(self) -> dict:
-        """phase name -> grouped launch, for the launches timed one by one (Y, T_hat, T_tilde)."""
+        """phase name -> grouped launch, for the launches timed one by one (Y, T_hat, T_tilde and
+        the streamed Z)."""
         out = {}
-        for prefix, multi in (("y", self.g_y), ("that", self.g_that), ("ttilde", self.g_ttil)):
+        fams = [("y", self.g_y), ("that", self.g_that), ("ttilde", self.g_ttil)]
+        if self.z_streamed:
+            fams.append(("z", self.g_z))
+        for prefix, multi in fams:
             out.update(zip(self._tags(prefix, multi), multi.groups))
         return out
 
@@ -692,9 +700,14 @@ class SweepEngine:
         from .autotune import tune_engine
         return tune_engine(self, reps=reps, cache_path=cache_path, verbose=verbose, write=write)
 
+    @property
+    def z_streamed(self) -> bool:
+        return isinstance(self.g_z, StreamMulti)
+
     def stream_launch_order(self):
         """phase names of the streamed launches in the order a sweep issues them"""
-        return self._tags("y", self.g_y) + self._tags("that", self.g_that) + self._tags("ttilde", self.g_ttil)
+        z = self._tags("z", self.g_z) if self.z_streamed else []
+        return self._tags("y", self.g_y) + self._tags("that", self.g_that) + z + self._tags("ttilde", self.g_ttil)
 
     def _run_marked(self, prefix, multi, stream):
         # one marker per launch, so each launch is timed on its own
@@ -803,8 +816,11 @@ class SweepEngine:
             L.call("fh_krp", self.A_view(c).data_ptr(), cl.sp, self.B_view(c).data_ptr(), cl.sp, cl.n, cl.s,
                    cl.sp, self.Lk.data_ptr() + self.Lk.element_size() * cl.loff, self.xdt,
                    self.pscale.data_ptr() + 4 * cl.psoff[0] if self.f8 else None, st)
-        mk("z")
-        self.g_z.run(stream)
+        if self.z_streamed:
+            self._run_marked("z", self.g_z, stream)
+        else:
+            mk("z")
+            self.g_z.run(stream)
         mk("svd_term")
         self.g_svd.run(stream)
         # meta-embedding polar

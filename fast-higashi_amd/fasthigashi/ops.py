@@ -153,13 +153,33 @@ def stream_bm(M: int) -> int:
 
 class StreamGemmGroup:
     """Grouped launch of the streaming bf16 GEMM (both operands k-contiguous, persistent
-    workgroups, global->LDS ring); tiles bm x bn: bm in STREAM_BM with bn = 128, or 128 x 256."""
+    workgroups, global->LDS ring); tiles bm x bn: bm in STREAM_BM with bn = 128, or 128 x 256.
+    a_mn: A stored m-contiguous (sAm = 1; read transposed from LDS), fp32 output, 128 x 160."""
 
     def __init__(self, descs: Sequence[L.GemmDesc], bm: Optional[int] = None, grid: int = 0, device="cuda",
-                 bn: int = 128, fp8: bool = False):
+                 bn: int = 128, fp8: bool = False, a_mn: bool = False):
         assert len(descs) > 0
         self.fp8 = fp8
+        self.a_mn = a_mn
         v = 16 if fp8 else 8             # elements per 16-byte chunk
+        if a_mn:
+            assert not fp8 and (bm in (None, 128)) and bn == 160, "m-contiguous A: bf16, 128 x 160 tiles"
+            for d in descs:
+                assert d.sAm == 1 and d.sBk == 1, "m-contiguous A with k-contiguous B"
+                assert d.sAk % 8 == 0 and d.sAk >= d.M and d.M % 8 == 0 and d.K % 8 == 0, \
+                    "A: k stride %d, M %d, K %d (multiples of 8)" % (d.sAk, d.M, d.K)
+                assert d.sBn % 8 == 0 and d.sBn >= d.K, "B leading dim %d (K %d)" % (d.sBn, d.K)
+                assert d.batch == 1 or (d.sAb % 8 == 0 and d.sBb % 8 == 0), "batch strides"
+                assert d.A % 16 == 0 and d.B % 16 == 0, "operand pointers 16-byte aligned"
+                assert d.beta == 0.0 and d.diag == 0.0 and d.flags == 0, "m-contiguous A: fp32 output, no flags"
+                assert not (d.scale_a or d.scale_b), "the streaming GEMM takes no scales"
+            self.bm, self.bn, self.grid = 128, 160, grid
+            arr = (L.GemmDesc * len(descs))(*descs)
+            self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), 128, 160)
+            self.n = len(descs)
+            self.descs = list(arr)
+            self.table = _upload(arr, device)
+            return
         for d in descs:
             assert d.sAk == 1 and d.sBk == 1, "streaming GEMM needs k-contiguous A and B"
             # 16-byte chunks along k: rows readable (and zero) up to K rounded to a whole chunk
@@ -181,8 +201,9 @@ class StreamGemmGroup:
         self.table = _upload(arr, device)
 
     def run(self, stream=None):
-        L.call("fh_gemm_stream8_launch" if self.fp8 else "fh_gemm_stream_launch", self.bm, self.bn,
-               self.table.data_ptr(), self.n, self.total, self.grid, stream_handle(stream))
+        name = ("fh_gemm_stream_mn_launch" if self.a_mn else
+                "fh_gemm_stream8_launch" if self.fp8 else "fh_gemm_stream_launch")
+        L.call(name, self.bm, self.bn, self.table.data_ptr(), self.n, self.total, self.grid, stream_handle(stream))
 
 
 class QuantRowsGroup:

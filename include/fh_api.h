@@ -109,6 +109,15 @@ int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhGemmDesc* de
  * when K % 8 != 0; flags 0 or FH_GEMM_OUT_BF16, beta 0. */
 int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                           int grid, fh_stream_t stream);
+/* The streaming GEMM with A stored m-contiguous (element (m, k) at A[m + k sAk], sAm = 1) and B
+ * k-contiguous, bf16 operands, fp32 output C = alpha A B^T; tiles (bm, bn) = (128, 160).  A's
+ * k-rows go to LDS as they lie in memory and the MFMA operands are read back transposed
+ * (ds_read_b64_tr_b16).  Contract: sAk and B's leading dim multiples of 8, M a multiple of 8,
+ * 16-byte aligned bases, K a multiple of 8, flags 0, beta 0.  The contraction
+ * SVD_term += lhs (U^T X) of parafac2_intergrative.py:419-427 in its factored order, Z_c =
+ * T_hat_c^T L_c^T, reading T_hat_c (positions x cells, cells contiguous) as stored. */
+int fh_gemm_stream_mn_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
+                             int grid, fh_stream_t stream);
 /* The same streaming GEMM with FP8 (OCP e4m3) operands: MFMA 16x16x128 f8f6f4, fp32 accumulate,
  * k-steps of 128 elements (one 128-byte row chunk, as for bf16).  C = alpha A B^T; the operands'
  * scales are applied by the caller outside the GEMM (scale_a / scale_b must be NULL).  Contract as

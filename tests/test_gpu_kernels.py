@@ -328,6 +328,38 @@ def test_gemm_stream_wide_tile(cuda):
         assert bool((C[:, M:].float() == 7.0).all()), case
 
 
+def test_gemm_stream_m_contiguous_a(cuda):
+    """streaming GEMM with A stored m-contiguous (element (m, k) at m + k lda, read from LDS
+    with ds_read_b64_tr_b16), 128 x 160 tiles, fp32 output: against the FP64 product of the same
+    bf16 values, for M not a multiple of 128 (chunks past M zero-filled), K below, at and above
+    one 64-wide step and not a multiple of 64 (k-rows past K zero-filled), N below and above
+    one 160-wide tile, batches, lda > M; rows of C beyond M untouched."""
+    from fasthigashi import ops
+    gen = torch.Generator().manual_seed(43)
+    cases = [(256, 160, 640, 1), (200, 77, 136, 2), (128, 150, 200, 1), (1000, 33, 1000, 1), (8, 5, 8, 3),
+             (384, 300, 64, 1), (136, 160, 72, 2)]
+    built = []
+    for (M, N, K, b) in cases:
+        lda = M + 24
+        A = torch.randn(b, K, lda, generator=gen).to(torch.bfloat16).to(cuda)      # [b][k][m]
+        Bm = torch.randn(b, N, K, generator=gen).to(torch.bfloat16).to(cuda)      # [b][n][k]
+        C = torch.full((b, M + 3, N), 7.0, dtype=torch.float32, device=cuda)
+        d = ops.make_desc(A.data_ptr(), Bm.data_ptr(), C.data_ptr(), M, N, K, 1, lda, 1, K, N, 1, b,
+                          K * lda, N * K, (M + 3) * N)
+        ref = A[:, :, :M].double().cpu().transpose(1, 2) @ Bm.double().cpu().transpose(1, 2)
+        built.append((d, C, ref, M, (A, Bm)))
+    for grid in (0, 5):
+        for _, C, _, _, _ in built:
+            C.fill_(7.0)
+        ops.StreamGemmGroup([x[0] for x in built], bm=128, bn=160, grid=grid, device=cuda, a_mn=True).run()
+        torch.cuda.synchronize()
+        for (d, C, ref, M, _), case in zip(built, cases):
+            got = C[:, :M].double().cpu()
+            err = (got - ref).abs().max().item() / ref.abs().max().item()
+            assert err < 1e-5, (grid, case, err)
+            assert bool((C[:, M:] == 7.0).all()), (grid, case)
+
+
 def test_gemm_stream_matches_tiled_gemm(cuda):
     """same descriptors through the streaming kernel and the general tiled bf16 GEMM."""
     from fasthigashi import ops, _lib as L
