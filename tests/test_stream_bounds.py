@@ -1,0 +1,123 @@
+"""Global-memory bounds of the streaming GEMM (fh_gemm_stream.hip), checked on the CPU.
+
+The kernel's load addresses are restated here (the tile walk of gemm_stream_kernel, load_ref
+and issue_stage: every load instruction of every lane of every wave, for every tile of every
+descriptor, at the first and the last k-step, where the address extremes lie)
+and every 16-byte access that is not redirected to the zero block must fall inside the
+tensor its operand points into.  Addresses grow with the batch, tile row and tile column
+indices, and only a last (partial) tile clamps or drops rows, so the first and the last two
+tiles along each of them cover the extremes.  The descriptor tables are the sweep engine's
+own (BF16 and FP8 engines of the c2 chromosome layout at a small cell count, built on the
+CPU).
+"""
+import numpy as np
+import pytest
+
+# (bm, bn, m-contiguous A) -> (WM_T, WN_T, WGM, WGN, STAGES), as stream_dispatch picks them
+CFG = {(64, 128, False): (2, 4, 2, 2, 4), (96, 128, False): (3, 4, 2, 2, 4), (128, 128, False): (4, 4, 2, 2, 4),
+       (160, 128, False): (5, 4, 2, 2, 4), (128, 256, False): (4, 4, 2, 4, 3), (128, 160, True): (2, 10, 4, 1, 4)}
+
+
+def amn_swz(r):
+    return 2 * ((r & 3) | ((r >> 2) & 4))
+
+
+def _tensors(eng):
+    import torch
+    out = []
+    for v in vars(eng).values():
+        if isinstance(v, torch.Tensor) and v.numel() > 0:
+            out.append((v.data_ptr(), v.data_ptr() + v.numel() * v.element_size()))
+    return sorted(out)
+
+
+def _owner(intervals, p):
+    for lo, hi in intervals:
+        if lo <= p < hi:
+            return lo, hi
+    raise AssertionError("operand pointer %#x is in no engine tensor" % p)
+
+
+def _check_group(g, fp8, intervals):
+    WM_T, WN_T, WGM, WGN, STAGES = CFG[(g.bm, g.bn, bool(getattr(g, "a_mn", False)))]
+    amn = bool(getattr(g, "a_mn", False))
+    NW = WGM * WGN
+    BM, BN = 16 * WM_T * WGM, 16 * WN_T * WGN
+    DA, DB = BM // 8, BN // 8
+    D = (DA + DB) // NW
+    ES = 1 if fp8 else 2
+    EK = 128 // ES
+    wave = np.arange(NW)[:, None, None]
+    lane = np.arange(64)[None, :, None]
+    q = np.arange(D)[None, None, :]
+    i = wave + NW * q
+    isA = i < DA
+    blk = np.where(isA, i, i - DA)
+    checked = 0
+    for d in g.descs:
+        M, N, K = d.M, d.N, d.K
+        a_lo, a_hi = _owner(intervals, d.A)
+        b_lo, b_hi = _owner(intervals, d.B)
+        nk = -(-K // EK)
+        nv_last = ((K - EK * (nk - 1)) * ES + 15) // 16
+        krem = K - EK * (nk - 1)
+        tm_n, tn_n = -(-M // BM), -(-N // BN)
+        ends = lambda n: sorted({0, max(n - 2, 0), n - 1})
+        for b in ends(d.batch):
+            for tm in ends(tm_n):
+                for tn in ends(tn_n):
+                    m0, n0 = tm * BM, tn * BN
+                    baseA = d.A + ES * (b * d.sAb + m0 * d.sAm)
+                    baseB = d.B + ES * (b * d.sBb + n0 * d.sBn)
+                    rows_a, rows_b = M - m0, N - n0
+                    for kk in sorted({0, nk - 1}):
+                        last = kk == nk - 1
+                        # stage loads
+                        if amn:
+                            rowA = blk * 4 + (lane >> 4)
+                            cA = (lane & 15) ^ amn_swz(rowA)
+                            zA = (m0 + 8 * cA >= M) | (last & (rowA >= krem))
+                            addrA = baseA + ES * (rowA * d.sAk + 8 * cA) + kk * EK * ES * d.sAk
+                        else:
+                            rowA = blk * 8 + (lane >> 3)
+                            cA = (lane & 7) ^ (rowA & 7)
+                            rrA = np.minimum(rowA, rows_a - 1)
+                            zA = last & (nv_last < 8) & (((lane & 7) ^ (lane >> 3)) >= nv_last) | np.zeros_like(rowA, bool)
+                            addrA = baseA + ES * rrA * d.sAm + 16 * cA + kk * 128
+                        rowB = blk * 8 + (lane >> 3)
+                        cB = (lane & 7) ^ (rowB & 7)
+                        rrB = np.minimum(rowB, rows_b - 1)
+                        zB = last & (nv_last < 8) & (((lane & 7) ^ (lane >> 3)) >= nv_last) | np.zeros_like(rowB, bool)
+                        addrB = baseB + ES * rrB * d.sBn + 16 * cB + kk * 128
+                        for use, addr, z, lo, hi in ((isA, addrA, zA, a_lo, a_hi), (~isA, addrB, zB, b_lo, b_hi)):
+                            sel = use & ~z
+                            if sel.any():
+                                a = addr[np.broadcast_to(sel, addr.shape)] if addr.shape == sel.shape else \
+                                    np.broadcast_to(addr, sel.shape)[sel]
+                                assert a.min() >= lo and a.max() + 16 <= hi, (g.bm, g.bn, amn, tm, tn, kk)
+                                checked += a.size
+    return checked
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+def test_stream_gemm_loads_in_bounds(dtype):
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fast-higashi_amd"))
+    from fasthigashi import synthetic as syn, tiling, _lib as L
+    from fasthigashi.engine import SweepEngine
+    try:
+        L.lib()
+    except L.NativeError:
+        pytest.skip("libfh_hip.so not built")
+    wl = syn.WORKLOADS["c2"]
+    n_bins = list(wl.bins)
+    tiles = [tiling.tile_plan(n, tiling.bs_bin_for(n, wl.resolution), wl.off_diag) for n in n_bins]
+    sizes = [tiling.size_of(n, wl.dim1, wl.resolution, wl.rank) for n in n_bins]
+    eng = SweepEngine(tiles, n_bins, sizes, 200, wl.rank, dtype=dtype, device="cpu")
+    intervals = _tensors(eng)
+    groups = eng.launch_groups()
+    assert any(getattr(g, "a_mn", False) for g in groups.values())
+    total = 0
+    for g in groups.values():
+        total += _check_group(g, g.fp8, intervals)
+    assert total > 0
