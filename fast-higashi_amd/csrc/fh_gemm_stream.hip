@@ -21,7 +21,8 @@
 // the instruction cache.
 //
 // A-m-contiguous mode (AMN, bf16, 128 x 160 tiles): A is stored k-major with m contiguous
-// (element (m, k) at m + k sAk; Z = T_hat^T L^T reads T_hat so).  A stage then holds A as 64
+// (element (m, k) at m + k sAk; Z = T_hat^T L^T reads T_hat so, T_hat^T = X^T P reads X so).
+// Output: fp32 row-major, or bf16 transposed (m contiguous, written back as T_hat's rows).  A stage then holds A as 64
 // k-rows of BM * 2 = 256 bytes; a 1 KB load fills 4 of them (lane l: position l % 16 of row
 // 4 q + l / 16), and chunk c of row r sits at position c ^ amn_swz(r).  The MFMA operand of
 // a 16-row m block, 8 consecutive k per lane, comes from two ds_read_b64_tr_b16: in each
@@ -410,6 +411,40 @@ __device__ __forceinline__ void store_tile(const StoreRef& sr, floatx4 (&acc)[C:
   }
 }
 
+// Transposed bf16 output of an AMN tile (C element (m, n) at C + m + n sCn, m contiguous):
+// per 16-column block j, the wave stages its 16 n-rows of 16 WM_T m values (bf16) and writes
+// each n-row's 16-byte chunks of 8 consecutive m.  T_hat_i = (X_i^T P_i)^T stored as T_hat's
+// rows (bin, p) with cells contiguous.  rows (M - m0) is a multiple of 8, so a chunk is
+// wholly inside or outside; columns from cols (N - n0) on are not written.
+template <class C>
+__device__ __forceinline__ void store_tile_tb16(const StoreRef& sr, floatx4 (&acc)[C::WM_T][C::WN_T], int wm,
+                                                int wn, int lane, uint8_t* so) {
+  constexpr int WM_T = C::WM_T, WN_T = C::WN_T, MW = 16 * WM_T;
+  static_assert(16 * MW * 2 <= 8 * kStageRow && MW % 8 == 0, "staging of one 16-column block");
+  const float alpha = sr.alpha;
+  const int64_t sCn = sr.sCn, rows = sr.rows, cols = sr.cols;
+  const int mb = wm * MW;
+  uint16_t* st = (uint16_t*)so;
+#pragma unroll
+  for (int j = 0; j < WN_T; ++j) {
+    const int nb = wn * WN_T * 16 + j * 16;
+    if (nb >= cols) break;                         // uniform
+    // lane l holds (m = mb + 16 i + l % 16, n = nb + 4 (l / 16) + r)
+#pragma unroll
+    for (int i = 0; i < WM_T; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        st[(4 * (lane >> 4) + r) * MW + 16 * i + (lane & 15)] = fh_f32_to_bf16_bits(acc[i][j][r] * alpha);
+#pragma unroll
+    for (int h = 0; h < (16 * MW / 8) / 64; ++h) {
+      const int e = lane + 64 * h, n = e / (MW / 8), ch = e % (MW / 8);
+      const u32x4 v = *(const u32x4*)(st + n * MW + ch * 8);
+      const int64_t gn = nb + n, gm = mb + ch * 8;
+      if (gn < cols && gm < rows) *(gu32x4*)((gu16*)sr.C + gn * sCn + gm) = v;
+    }
+  }
+}
+
 // One barrier per k-step: at step k every wave has waited for its loads of stage k % STAGES
 // and finished step k - 1, so the stage read at step k - 1 is refilled with step
 // k + STAGES - 1 (the load cursor runs STAGES - 1 steps ahead, through zero blocks past the
@@ -465,7 +500,9 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
                               wm, wn, lane);
     if (++ck >= sr.nk) {
       if constexpr (AMN) {
-        store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);   // fp32 output only (host-checked)
+        // fp32 output row-major (Z), bf16 output transposed (T_hat); host-checked
+        if (sr.bf16) store_tile_tb16<C>(sr, acc, wm, wn, lane, SO[wave]);
+        else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);
       } else {
         if (sr.bf16) store_tile<2, C>(sr, acc, wm, wn, lane, SO[wave]);
         else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);

@@ -244,7 +244,11 @@ class SweepEngine:
 
     def __init__(self, tiles: Sequence[Sequence[TileSpec]], n_bins: Sequence[int], sizes: Sequence[int],
                  n_cells: int, rank: int, dtype: str = "fp32", device="cuda", names=None,
-                 ns_iter: int = 48, ns_tol: float = 1e-20, meta_ksplit: int = 32, n_iter_max_hist: int = 512):
+                 ns_iter: int = 48, ns_tol: float = 1e-20, meta_ksplit: int = 32, n_iter_max_hist: int = 512,
+                 t_hat_from_x: Optional[bool] = None):
+        """t_hat_from_x (bf16 path): compute T_hat_i^T = X_i^T P_i from X as stored (m-contiguous
+        streaming mode) instead of from a positions-contiguous copy X_i^T; the copy is as large as
+        X, the direct form is ~25 % slower.  None: the copy when it fits in device memory."""
         assert dtype in ("fp32", "bf16", "fp8")
         self.dtype = dtype
         self.f8 = dtype == "fp8"
@@ -297,10 +301,11 @@ class SweepEngine:
         self.Y = z(yoff)
         self.temp = z(yoff)
         self.P = z(yoff)
-        # bf16 path: k-contiguous copies for T_hat = P^T X (per bin: X_i^T cells x positions,
-        # P_i^T sp x positions, positions padded to a multiple of KPAD)
+        # bf16 path: the k-contiguous copy P_i^T (sp x positions, positions padded to a multiple
+        # of KPAD) for T_hat_i^T = X_i^T P_i, which reads X as stored; fp8 path: also X_i^T
         self.P16 = None
-        self.Xt = self.P16t = None
+        self.P16t = None
+        self.Xt = None
         self.X8 = self.X8t = self.P8t = None
         if dtype in ("bf16", "fp8"):
             kp = KPAD8 if self.f8 else KPAD
@@ -314,6 +319,12 @@ class SweepEngine:
                     xt += t.rows * self.Npad * wp
                     pt += t.rows * cl.sp * wp
             self.P16t = z(pt, dt=torch.bfloat16)
+            if dtype == "bf16":
+                if t_hat_from_x is None:
+                    t_hat_from_x = not self._fits(2 * xt + 2 * sum(cl.n * cl.sp for cl in self.chroms) * self.Npad
+                                                  + 5 * 8 * sum(cl.n * cl.s * rup(cl.s, 4) for cl in self.chroms))
+                if not t_hat_from_x:
+                    self.Xt = z(xt, dt=torch.bfloat16)
             if self.f8:
                 # e4m3 copies: X (positions x cells) for Y = X C, X_i^T per bin for T_hat = P^T X;
                 # one scale per row (position) of X, per row of C^T and per row of P^T
@@ -330,8 +341,6 @@ class SweepEngine:
                         cl.psoff.append(prow)
                         prow += t.rows * cl.sp
                 self.pscale = z(max(prow, 1))
-            else:
-                self.Xt = z(xt, dt=torch.bfloat16)
         self.That = z(self.Ttot, self.Npad, dt=self.tdtype)
         self.Lk = z(loff, dt=self.tdtype)
         self.Zt = z(self.Npad, self.Dld)
@@ -419,11 +428,17 @@ class SweepEngine:
                 self.X_tile(c, t)[:, :, :self.N] = arr.to(self.tdtype)
         self.finish_X()
 
+    def _fits(self, nbytes: int) -> bool:
+        """Whether nbytes more (plus 8 GB of headroom) fit in the device's free memory."""
+        if self.device.type != "cuda":
+            return True
+        free, _ = torch.cuda.mem_get_info(self.device)
+        return nbytes + (8 << 30) <= free
+
     def finish_X(self):
         """Squared norms of each chromosome's X (the reference's first-sweep ||X||^2); on the
-        bf16 path also the per-bin transposed copy X_i^T used by T_hat = P^T X; on the fp8 path
-        the e4m3 copies X8 and X8t (one scale per position row), after which the bf16 X is
-        released."""
+        bf16 path the per-bin copy X_i^T when T_hat is formed from it; on the fp8 path the e4m3
+        copies X8 and X8t (one scale per position row), after which the bf16 X is released."""
         if self.X is None:
             raise RuntimeError("X was already quantised and released (fp8 path)")
         if self.Xt is not None:
@@ -565,7 +580,8 @@ class SweepEngine:
             self.g_x8t = ops.TransposeGroup(x8t, dev)
             self.g_pt = ops.TransposeGroup(pt, dev)
             self.g_p8 = ops.QuantRowsGroup(q, dev)
-        elif self.dtype == "bf16":
+        elif self.dtype == "bf16" and self.Xt is not None:
+            # T_hat_i = P_i^T X_i from the k-contiguous copies P_i^T and X_i^T
             xt = []
             pt = []
             for c, cl in enumerate(self.chroms):
@@ -584,6 +600,22 @@ class SweepEngine:
                                         Np * wp, cl.sp * Np, flags=out16))
             self.g_xt = ops.TransposeGroup(xt, dev)
             self.g_pt = ops.TransposeGroup(pt, dev)
+        elif self.dtype == "bf16":
+            # T_hat_i^T = X_i^T P_i: A = X_i (positions x cells) read cells-contiguous as stored,
+            # B = P_i^T (k-contiguous copy), the bf16 output written transposed into T_hat's rows
+            pt = []
+            for c, cl in enumerate(self.chroms):
+                for t, tl in enumerate(cl.tiles):
+                    wp = rup(tl.width, KPAD)
+                    xo = (cl.xrow + cl.trow[t]) * Np
+                    po = cl.yoff + cl.trow[t] * cl.sp
+                    pt.append((f4(self.P, po), F32, self.P16t.data_ptr() + 2 * cl.ptoff[t], BF16, tl.width,
+                               cl.s, cl.sp, wp, tl.rows, tl.width * cl.sp, cl.sp * wp))
+                    d8.append(make_desc(Xp + 2 * xo, self.P16t.data_ptr() + 2 * cl.ptoff[t],
+                                        Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
+                                        Np, cl.s, tl.width, 1, Np, 1, wp, 1, Np, tl.rows, tl.width * Np,
+                                        cl.sp * wp, cl.sp * Np, flags=out16))
+            self.g_pt = ops.TransposeGroup(pt, dev)
         else:
             for c, cl in enumerate(self.chroms):
                 for t, tl in enumerate(cl.tiles):
@@ -592,7 +624,9 @@ class SweepEngine:
                                         Thp + xe * (cl.toff + tl.start * cl.sp) * Np,
                                         cl.s, Np, tl.width, 1, cl.sp, Np, 1, Np, 1, tl.rows, tl.width * cl.sp,
                                         tl.width * Np, cl.sp * Np, flags=out16))
-        if lowp:
+        if self.dtype == "bf16" and self.Xt is None:
+            self.g_that = StreamMulti([ops.StreamGemmGroup(d8, bm=128, bn=160, device=dev, a_mn=True)])
+        elif lowp:
             # streaming kernel: one persistent launch per tile height (64 .. 160 rows >= s)
             by = {}
             for d in d8:

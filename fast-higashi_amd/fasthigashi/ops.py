@@ -154,7 +154,8 @@ def stream_bm(M: int) -> int:
 class StreamGemmGroup:
     """Grouped launch of the streaming bf16 GEMM (both operands k-contiguous, persistent
     workgroups, global->LDS ring); tiles bm x bn: bm in STREAM_BM with bn = 128, or 128 x 256.
-    a_mn: A stored m-contiguous (sAm = 1; read transposed from LDS), fp32 output, 128 x 160."""
+    a_mn: A stored m-contiguous (sAm = 1; read transposed from LDS), 128 x 160 tiles; fp32 output
+    (any layout) or bf16 output stored m-contiguous (sCm = 1)."""
 
     def __init__(self, descs: Sequence[L.GemmDesc], bm: Optional[int] = None, grid: int = 0, device="cuda",
                  bn: int = 128, fp8: bool = False, a_mn: bool = False):
@@ -163,15 +164,20 @@ class StreamGemmGroup:
         self.a_mn = a_mn
         v = 16 if fp8 else 8             # elements per 16-byte chunk
         if a_mn:
-            assert not fp8 and (bm in (None, 128)) and bn == 160, "m-contiguous A: bf16, 128 x 160 tiles"
+            assert not fp8 and (bm in (None, 128)) and bn == 160, "m-contiguous A: bf16 operands, 128 x 160 tiles"
             for d in descs:
                 assert d.sAm == 1 and d.sBk == 1, "m-contiguous A with k-contiguous B"
-                assert d.sAk % 8 == 0 and d.sAk >= d.M and d.M % 8 == 0 and d.K % 8 == 0, \
-                    "A: k stride %d, M %d, K %d (multiples of 8)" % (d.sAk, d.M, d.K)
-                assert d.sBn % 8 == 0 and d.sBn >= d.K, "B leading dim %d (K %d)" % (d.sBn, d.K)
+                assert d.sAk % 8 == 0 and d.sAk >= d.M and d.M % 8 == 0, \
+                    "A: k stride %d, M %d (multiples of 8)" % (d.sAk, d.M)
+                # B's 16-byte chunks along k: rows readable (and zero) up to K rounded to a chunk
+                assert d.sBn % 8 == 0 and d.sBn >= -(-d.K // 8) * 8, "B leading dim %d (K %d)" % (d.sBn, d.K)
                 assert d.batch == 1 or (d.sAb % 8 == 0 and d.sBb % 8 == 0), "batch strides"
                 assert d.A % 16 == 0 and d.B % 16 == 0, "operand pointers 16-byte aligned"
-                assert d.beta == 0.0 and d.diag == 0.0 and d.flags == 0, "m-contiguous A: fp32 output, no flags"
+                assert d.beta == 0.0 and d.diag == 0.0 and d.flags in (0, L.GEMM_OUT_BF16), "flags 0 or OUT_BF16"
+                if d.flags == L.GEMM_OUT_BF16:
+                    # bf16 output is written transposed: element (m, n) at C + m + n sCn
+                    assert d.sCm == 1 and d.sCn % 8 == 0 and d.sCn >= d.M and d.C % 16 == 0 and \
+                        (d.batch == 1 or d.sCb % 8 == 0), "bf16 output: m-contiguous C (sCm 1, sCn % 8 == 0)"
                 assert not (d.scale_a or d.scale_b), "the streaming GEMM takes no scales"
             self.bm, self.bn, self.grid = 128, 160, grid
             arr = (L.GemmDesc * len(descs))(*descs)

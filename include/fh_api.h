@@ -113,9 +113,12 @@ int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_des
  * k-contiguous, bf16 operands, fp32 output C = alpha A B^T; tiles (bm, bn) = (128, 160).  A's
  * k-rows go to LDS as they lie in memory and the MFMA operands are read back transposed
  * (ds_read_b64_tr_b16).  Contract: sAk and B's leading dim multiples of 8, M a multiple of 8,
- * 16-byte aligned bases, K a multiple of 8, flags 0, beta 0.  The contraction
- * SVD_term += lhs (U^T X) of parafac2_intergrative.py:419-427 in its factored order, Z_c =
- * T_hat_c^T L_c^T, reading T_hat_c (positions x cells, cells contiguous) as stored. */
+ * 16-byte aligned bases, K a multiple of 8, beta 0; flags 0 (fp32 C, any strides) or
+ * FH_GEMM_OUT_BF16 (bf16 C stored m-contiguous: sCm = 1, sCn a multiple of 8).  The
+ * contraction SVD_term += lhs (U^T X) of parafac2_intergrative.py:419-427 in its factored
+ * order, Z_c = T_hat_c^T L_c^T, reading T_hat_c (positions x cells, cells contiguous) as
+ * stored; and projected = U^T X (:426) as T_hat_i^T = X_i^T P_i, reading X_i as stored and
+ * writing T_hat_i's rows. */
 int fh_gemm_stream_mn_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                              int grid, fh_stream_t stream);
 /* The same streaming GEMM with FP8 (OCP e4m3) operands: MFMA 16x16x128 f8f6f4, fp32 accumulate,

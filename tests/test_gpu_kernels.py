@@ -333,8 +333,9 @@ def test_gemm_stream_m_contiguous_a(cuda):
     with ds_read_b64_tr_b16), 128 x 160 tiles, fp32 output: against the FP64 product of the same
     bf16 values, for M not a multiple of 128 (chunks past M zero-filled), K below, at and above
     one 64-wide step and not a multiple of 64 (k-rows past K zero-filled), N below and above
-    one 160-wide tile, batches, lda > M; rows of C beyond M untouched."""
-    from fasthigashi import ops
+    one 160-wide tile, batches, lda > M; rows of C beyond M untouched.  Then the same products
+    with bf16 output stored m-contiguous (the transposed epilogue), within one bf16 rounding."""
+    from fasthigashi import ops, _lib as L
     gen = torch.Generator().manual_seed(43)
     cases = [(256, 160, 640, 1), (200, 77, 136, 2), (128, 150, 200, 1), (1000, 33, 1000, 1), (8, 5, 8, 3),
              (384, 300, 64, 1), (136, 160, 72, 2)]
@@ -358,6 +359,21 @@ def test_gemm_stream_m_contiguous_a(cuda):
             err = (got - ref).abs().max().item() / ref.abs().max().item()
             assert err < 1e-5, (grid, case, err)
             assert bool((C[:, M:] == 7.0).all()), (grid, case)
+    # bf16 output written transposed (element (m, n) at n * ldc + m), the T_hat^T = X^T P use
+    tb = []
+    for (d, C, ref, M, keep), (_, N, K, b) in zip(built, cases):
+        ldc = M + 16
+        Ct = torch.full((b, N + 2, ldc), 7.0, dtype=torch.bfloat16, device=cuda)
+        dt = ops.make_desc(d.A, d.B, Ct.data_ptr(), M, N, K, 1, d.sAk, 1, d.sBn, 1, ldc, b, d.sAb, d.sBb,
+                           (N + 2) * ldc, flags=L.GEMM_OUT_BF16)
+        tb.append((dt, Ct, ref, M, N, keep))
+    ops.StreamGemmGroup([x[0] for x in tb], bm=128, bn=160, device=cuda, a_mn=True).run()
+    torch.cuda.synchronize()
+    for (dt, Ct, ref, M, N, _), case in zip(tb, cases):
+        got = Ct[:, :N, :M].double().cpu().transpose(1, 2)
+        err = (got - ref).abs().max().item() / ref.abs().max().item()
+        assert err < 1e-2, ("bf16 transposed", case, err)
+        assert bool((Ct[:, N:, :].float() == 7.0).all()) and bool((Ct[:, :, M:].float() == 7.0).all()), case
 
 
 def test_gemm_stream_matches_tiled_gemm(cuda):

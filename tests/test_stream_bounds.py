@@ -7,8 +7,8 @@ and every 16-byte access that is not redirected to the zero block must fall insi
 tensor its operand points into.  Addresses grow with the batch, tile row and tile column
 indices, and only a last (partial) tile clamps or drops rows, so the first and the last two
 tiles along each of them cover the extremes.  The descriptor tables are the sweep engine's
-own (BF16 and FP8 engines of the c2 chromosome layout at a small cell count, built on the
-CPU).
+own (BF16 engines with T_hat formed from X_i^T and from X as stored, and the FP8 engine, of
+the c2 chromosome layout at a small cell count, built on the CPU).
 """
 import numpy as np
 import pytest
@@ -99,8 +99,8 @@ def _check_group(g, fp8, intervals):
     return checked
 
 
-@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
-def test_stream_gemm_loads_in_bounds(dtype):
+@pytest.mark.parametrize("dtype,from_x", [("bf16", False), ("bf16", True), ("fp8", None)])
+def test_stream_gemm_loads_in_bounds(dtype, from_x):
     import sys, os
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fast-higashi_amd"))
     from fasthigashi import synthetic as syn, tiling, _lib as L
@@ -113,7 +113,8 @@ def test_stream_gemm_loads_in_bounds(dtype):
     n_bins = list(wl.bins)
     tiles = [tiling.tile_plan(n, tiling.bs_bin_for(n, wl.resolution), wl.off_diag) for n in n_bins]
     sizes = [tiling.size_of(n, wl.dim1, wl.resolution, wl.rank) for n in n_bins]
-    eng = SweepEngine(tiles, n_bins, sizes, 200, wl.rank, dtype=dtype, device="cpu")
+    kw = {} if from_x is None else dict(t_hat_from_x=from_x)
+    eng = SweepEngine(tiles, n_bins, sizes, 200, wl.rank, dtype=dtype, device="cpu", **kw)
     intervals = _tensors(eng)
     groups = eng.launch_groups()
     assert any(getattr(g, "a_mn", False) for g in groups.values())
