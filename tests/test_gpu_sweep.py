@@ -81,6 +81,43 @@ def test_t_hat_from_x_matches_copy(cuda, tiny):
     assert np.max(np.abs(ha - hb) / np.abs(ha)) < 1e-6, (ha, hb)
 
 
+@pytest.mark.parametrize("dtype,tol", [("fp32", 1e-4), ("bf16", 1e-3)])
+def test_sweep_uneven_cell_chunks(cuda, dtype, tol):
+    """The batched uneven workload in miniature (RWR off as in its configuration): the oracle walks
+    the cells in random uneven chunks (parafac2_intergrative.py:345-386, 429-471, 488-529 sum
+    over them), the engine keeps every cell resident; per-iteration fit, x_U and x_V within the
+    path's tolerance, embeddings' canonical correlations > 0.999.  Three sweeps: without RWR this
+    miniature's ALS trajectory amplifies rounding differences ~3x per sweep (FP32 against the FP32
+    oracle: 1.7e-8 relative after the first sweep, 2.5e-4 after the sixth)."""
+    from oracle import ref_np as R
+    wl = syn.scaled(syn.WORKLOADS["c3"], n_cells=160, bins=[150, 120, 100], rank=32, off_diag=40, band=40,
+                    resolution=500_000, name="c3_tiny")
+    data, raw, nis = build_dataset(wl, bs_bin=60)
+    rng = np.random.default_rng(17)
+    for cd in data:
+        edges, k = [0], 0
+        while k < 160:
+            k = min(160, k + int(rng.integers(9, 42)))
+            edges.append(k)
+        cd.cell_chunks = [slice(a, b) for a, b in zip(edges[:-1], edges[1:])]
+    assert all(len(cd.cell_chunks) > 3 for cd in data)
+    F0, sizes = random_init(data, wl, wl.rank, seed=5)
+    n_it = 3
+    ref = R.fit(data, F0, n_it, 1, 0.0, np.float32 if dtype == "fp32" else np.float64)
+    eng = _engine(data, F0, sizes, wl.rank, dtype)
+    for _ in range(n_it):
+        eng.sweep()
+    torch.cuda.synchronize()
+    h = eng.history()
+    rec_ref = np.array(ref.rec_errors)
+    assert np.max(np.abs(h["rec_error"] - rec_ref) / rec_ref) < tol, (h["rec_error"], rec_ref)
+    assert np.max(np.abs(h["x_V"] - np.array(ref.x_V)) / np.abs(np.array(ref.x_V))) < tol
+    assert np.max(np.abs(h["x_U"] - np.array(ref.x_U)) / np.abs(np.array(ref.x_U))) < tol
+    A, B, D, M = eng.get_factors()
+    cc = canonical_corr(M, ref.factors.M)
+    assert cc.min() > 0.999, cc
+
+
 def test_sweep_fp64_oracle_fit(cuda, tiny):
     """Plain FP32 GPU path vs the FP64 oracle: per-iteration fit within 1e-4."""
     from oracle import ref_np as R
