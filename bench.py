@@ -61,11 +61,12 @@ def launch_work(eng, phase):
 
 def kernel_name(phase, head):
     tag = phase.split(":")[1].split("#")[0]
+    if tag.startswith("m"):
+        bm, bn = tag[1:].split("x")
+        return ("gemm_stream_kernel, bf16 operands, %sx%s tiles, global->LDS ring, the m-contiguous operand "
+                "(%s) read with ds_read_b64_tr_b16" % (bm, bn, "T_hat" if phase.startswith("z") else "X"))
     if tag.startswith("s"):
         bm, bn = tag[1:].split("x")
-        if phase.startswith("z"):
-            return ("gemm_stream_kernel, bf16 operands, %sx%s tiles, global->LDS ring, T_hat read cells-"
-                    "contiguous with ds_read_b64_tr_b16" % (bm, bn))
         opnd = "fp8 (e4m3)" if head == "fp8" and not phase.startswith("ttilde") else "bf16"
         return "gemm_stream_kernel, %s operands, %sx%s tiles, global->LDS ring" % (opnd, bm, bn)
     from fasthigashi import ops

@@ -20,7 +20,7 @@
 // run time, which keeps the kernel small enough that its once-per-tile output path stays in
 // the instruction cache.
 //
-// A-m-contiguous mode (AMN, bf16, 128 x 160 tiles): A is stored k-major with m contiguous
+// A-m-contiguous mode (AMN, bf16, 128 x 64 | 96 | 128 | 160 tiles): A is stored k-major with m contiguous
 // (element (m, k) at m + k sAk; Z = T_hat^T L^T reads T_hat so, T_hat^T = X^T P reads X so).
 // Output: fp32 row-major, or bf16 transposed (m contiguous, written back as T_hat's rows).  A stage then holds A as 64
 // k-rows of BM * 2 = 256 bytes; a 1 KB load fills 4 of them (lane l: position l % 16 of row
@@ -548,10 +548,17 @@ int stream_dispatch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int
   }
   if ((int64_t)grid > total_tiles) grid = (int)total_tiles;
   if constexpr (AMN) {
-    // 4 waves stacked along m (32 rows each), each over all 160 columns (fp32 output groups)
-    if (bm == 128 && bn == 160)
-      return launch_stream<StreamCfg<2, 10, 4, 1, 4>, false, true>(descs_dev, n_desc, total_tiles, grid, stream);
-    fh_set_error("gemm_stream_mn: unsupported tile %d x %d (128 x 160)", bm, bn);
+    // 4 waves stacked along m (32 rows each), each over all bn columns
+    if (bm == 128) {
+      switch (bn) {
+        case 64: return launch_stream<StreamCfg<2, 4, 4, 1, 4>, false, true>(descs_dev, n_desc, total_tiles, grid, stream);
+        case 96: return launch_stream<StreamCfg<2, 6, 4, 1, 4>, false, true>(descs_dev, n_desc, total_tiles, grid, stream);
+        case 128: return launch_stream<StreamCfg<2, 8, 4, 1, 4>, false, true>(descs_dev, n_desc, total_tiles, grid, stream);
+        case 160: return launch_stream<StreamCfg<2, 10, 4, 1, 4>, false, true>(descs_dev, n_desc, total_tiles, grid, stream);
+        default: break;
+      }
+    }
+    fh_set_error("gemm_stream_mn: unsupported tile %d x %d (128 x 64 | 96 | 128 | 160)", bm, bn);
     return FH_ERR_ARG;
   }
   if (bn == 128) {

@@ -7,7 +7,7 @@ covering s, one tile column over s, ...); `tune_engine` instead times every tile
 kernel family has on the engine's own data and keeps the fastest, group by group:
 
 * streaming bf16 / fp8 GEMM: (bm, bn) in (64 | 96 | 128 | 160, 128) and (128, 256) (the
-  m-contiguous-A mode of Z has the one shape 128 x 160 and is left as it is);
+  launches of the m-contiguous-A mode keep the width matched to s by rule);
 * tiled fp32 / bf16 MFMA GEMM: the seven shapes 128x128, 128x64, 64x64, 128x160, 64x128,
   256x128, 128x256.
 
@@ -74,7 +74,7 @@ def _group_key(family: str, g) -> str:
 
 def _candidates(g, device):
     if getattr(g, "a_mn", False):
-        return None, None                      # one tile shape (128 x 160) in this mode
+        return None, None                      # width matched to s by rule in this mode
     if isinstance(g, ops.StreamGemmGroup):
         return [((bm, bn), (lambda bm=bm, bn=bn: ops.StreamGemmGroup(g.descs, bm=bm, bn=bn, grid=g.grid,
                                                                       device=device, fp8=g.fp8)))

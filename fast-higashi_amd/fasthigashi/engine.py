@@ -90,6 +90,16 @@ def shape_for_m(M: int) -> int:
     return 0 if M <= 128 else 5
 
 
+def _amn_groups(descs, dev, split: bool):
+    """m-contiguous-A streaming launches: one per tile width covering the descriptors' N (= s)
+    when split, else one launch at the width covering the largest N."""
+    by = {}
+    top = ops.amn_bn(max(d.N for d in descs))
+    for d in descs:
+        by.setdefault(ops.amn_bn(d.N) if split else top, []).append(d)
+    return [ops.StreamGemmGroup(v, bm=128, bn=k, device=dev, a_mn=True) for k, v in sorted(by.items())]
+
+
 class StreamMulti:
     """The streaming-GEMM launches of one contraction (one per tile height)."""
 
@@ -625,7 +635,7 @@ class SweepEngine:
                                         cl.s, Np, tl.width, 1, cl.sp, Np, 1, Np, 1, tl.rows, tl.width * cl.sp,
                                         tl.width * Np, cl.sp * Np, flags=out16))
         if self.dtype == "bf16" and self.Xt is None:
-            self.g_that = StreamMulti([ops.StreamGemmGroup(d8, bm=128, bn=160, device=dev, a_mn=True)])
+            self.g_that = StreamMulti(_amn_groups(d8, dev, split=True))
         elif lowp:
             # streaming kernel: one persistent launch per tile height (64 .. 160 rows >= s)
             by = {}
@@ -636,13 +646,14 @@ class SweepEngine:
         else:
             self.g_that = MultiGemm(d8, self.xdt, lambda d: shape_for_m(d.M), dev)
         # (10) Zt_c = T_hat_c^T L_c^T.  bf16: streamed, T_hat (positions x cells, cells contiguous)
-        # read as stored (m-contiguous A, transposed LDS reads), one 128 x 160 launch
+        # read as stored (m-contiguous A, transposed LDS reads), tiles 128 x (64 | 96 | 128 | 160) by s
         d10 = []
         for cl in self.chroms:
             d10.append(make_desc(Thp + xe * cl.toff * Np, Lp + xe * cl.loff, f4(self.Zt, cl.doff),
                                  Np, cl.s, cl.n * cl.sp, 1, Np, 1, cl.n * cl.sp, Dld, 1))
         if lowp:
-            self.g_z = StreamMulti([ops.StreamGemmGroup(d10, bm=128, bn=160, device=dev, a_mn=True)])
+            # one launch: split by width it was slower at c2 (4.5 against 4.2 ms)
+            self.g_z = StreamMulti(_amn_groups(d10, dev, split=False))
         else:
             self.g_z = MultiGemm(d10, self.xdt, lambda d: shape_for_n(d.N), dev)
         # (11) SVD_term^T = Zt D_all^T
@@ -705,8 +716,11 @@ class SweepEngine:
 
     @staticmethod
     def group_tag(g) -> str:
-        """phase-name suffix of one grouped launch: s<bm>x<bn> (streaming) or t<shape> (tiled)."""
-        return ("s%dx%d" % (g.bm, g.bn)) if isinstance(g, ops.StreamGemmGroup) else ("t%d" % g.shape)
+        """phase-name suffix of one grouped launch: s<bm>x<bn> (streaming), m<bm>x<bn> (streaming,
+        m-contiguous A) or t<shape> (tiled)."""
+        if isinstance(g, ops.StreamGemmGroup):
+            return ("m%dx%d" if g.a_mn else "s%dx%d") % (g.bm, g.bn)
+        return "t%d" % g.shape
 
     def _tags(self, prefix, multi):
         """phase names of a family's launches: prefix:tag, with #2, #3 .. for repeated tags"""

@@ -143,6 +143,18 @@ class GemmGroup:
 STREAM_BM = (64, 96, 128, 160)
 
 
+# tile widths of the m-contiguous-A streaming mode (tile height 128)
+AMN_BN = (64, 96, 128, 160)
+
+
+def amn_bn(N: int) -> int:
+    """smallest m-contiguous-mode tile width covering N columns (the widest for N > 160)."""
+    for bn in AMN_BN:
+        if N <= bn:
+            return bn
+    return AMN_BN[-1]
+
+
 def stream_bm(M: int) -> int:
     """smallest streaming tile height covering M rows (largest for M > 160)."""
     for bm in STREAM_BM:
@@ -154,8 +166,8 @@ def stream_bm(M: int) -> int:
 class StreamGemmGroup:
     """Grouped launch of the streaming bf16 GEMM (both operands k-contiguous, persistent
     workgroups, global->LDS ring); tiles bm x bn: bm in STREAM_BM with bn = 128, or 128 x 256.
-    a_mn: A stored m-contiguous (sAm = 1; read transposed from LDS), 128 x 160 tiles; fp32 output
-    (any layout) or bf16 output stored m-contiguous (sCm = 1)."""
+    a_mn: A stored m-contiguous (sAm = 1; read transposed from LDS), 128 x bn tiles (bn in AMN_BN);
+    fp32 output (any layout) or bf16 output stored m-contiguous (sCm = 1)."""
 
     def __init__(self, descs: Sequence[L.GemmDesc], bm: Optional[int] = None, grid: int = 0, device="cuda",
                  bn: int = 128, fp8: bool = False, a_mn: bool = False):
@@ -164,7 +176,7 @@ class StreamGemmGroup:
         self.a_mn = a_mn
         v = 16 if fp8 else 8             # elements per 16-byte chunk
         if a_mn:
-            assert not fp8 and (bm in (None, 128)) and bn == 160, "m-contiguous A: bf16 operands, 128 x 160 tiles"
+            assert not fp8 and (bm in (None, 128)) and bn in AMN_BN, "m-contiguous A: bf16, 128 x (64|96|128|160)"
             for d in descs:
                 assert d.sAm == 1 and d.sBk == 1, "m-contiguous A with k-contiguous B"
                 assert d.sAk % 8 == 0 and d.sAk >= d.M and d.M % 8 == 0, \
@@ -179,9 +191,9 @@ class StreamGemmGroup:
                     assert d.sCm == 1 and d.sCn % 8 == 0 and d.sCn >= d.M and d.C % 16 == 0 and \
                         (d.batch == 1 or d.sCb % 8 == 0), "bf16 output: m-contiguous C (sCm 1, sCn % 8 == 0)"
                 assert not (d.scale_a or d.scale_b), "the streaming GEMM takes no scales"
-            self.bm, self.bn, self.grid = 128, 160, grid
+            self.bm, self.bn, self.grid = 128, bn, grid
             arr = (L.GemmDesc * len(descs))(*descs)
-            self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), 128, 160)
+            self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), 128, bn)
             self.n = len(descs)
             self.descs = list(arr)
             self.table = _upload(arr, device)

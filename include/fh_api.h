@@ -110,10 +110,11 @@ int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhGemmDesc* de
 int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                           int grid, fh_stream_t stream);
 /* The streaming GEMM with A stored m-contiguous (element (m, k) at A[m + k sAk], sAm = 1) and B
- * k-contiguous, bf16 operands, fp32 output C = alpha A B^T; tiles (bm, bn) = (128, 160).  A's
+ * k-contiguous, bf16 operands, C = alpha A B^T; tiles (bm, bn) = (128, 64 | 96 | 128 | 160).  A's
  * k-rows go to LDS as they lie in memory and the MFMA operands are read back transposed
  * (ds_read_b64_tr_b16).  Contract: sAk and B's leading dim multiples of 8, M a multiple of 8,
- * 16-byte aligned bases, K a multiple of 8, beta 0; flags 0 (fp32 C, any strides) or
+ * 16-byte aligned bases, B's rows zero from K to the next multiple of 8, beta 0; flags 0 (fp32
+ * C, any strides) or
  * FH_GEMM_OUT_BF16 (bf16 C stored m-contiguous: sCm = 1, sCn a multiple of 8).  The
  * contraction SVD_term += lhs (U^T X) of parafac2_intergrative.py:419-427 in its factored
  * order, Z_c = T_hat_c^T L_c^T, reading T_hat_c (positions x cells, cells contiguous) as

@@ -328,12 +328,13 @@ def test_gemm_stream_wide_tile(cuda):
         assert bool((C[:, M:].float() == 7.0).all()), case
 
 
-def test_gemm_stream_m_contiguous_a(cuda):
+@pytest.mark.parametrize("bn", [64, 96, 128, 160])
+def test_gemm_stream_m_contiguous_a(cuda, bn):
     """streaming GEMM with A stored m-contiguous (element (m, k) at m + k lda, read from LDS
-    with ds_read_b64_tr_b16), 128 x 160 tiles, fp32 output: against the FP64 product of the same
+    with ds_read_b64_tr_b16), 128 x bn tiles, fp32 output: against the FP64 product of the same
     bf16 values, for M not a multiple of 128 (chunks past M zero-filled), K below, at and above
     one 64-wide step and not a multiple of 64 (k-rows past K zero-filled), N below and above
-    one 160-wide tile, batches, lda > M; rows of C beyond M untouched.  Then the same products
+    one tile width, batches, lda > M; rows of C beyond M untouched.  Then the same products
     with bf16 output stored m-contiguous (the transposed epilogue), within one bf16 rounding."""
     from fasthigashi import ops, _lib as L
     gen = torch.Generator().manual_seed(43)
@@ -352,7 +353,7 @@ def test_gemm_stream_m_contiguous_a(cuda):
     for grid in (0, 5):
         for _, C, _, _, _ in built:
             C.fill_(7.0)
-        ops.StreamGemmGroup([x[0] for x in built], bm=128, bn=160, grid=grid, device=cuda, a_mn=True).run()
+        ops.StreamGemmGroup([x[0] for x in built], bm=128, bn=bn, grid=grid, device=cuda, a_mn=True).run()
         torch.cuda.synchronize()
         for (d, C, ref, M, _), case in zip(built, cases):
             got = C[:, :M].double().cpu()
@@ -367,7 +368,7 @@ def test_gemm_stream_m_contiguous_a(cuda):
         dt = ops.make_desc(d.A, d.B, Ct.data_ptr(), M, N, K, 1, d.sAk, 1, d.sBn, 1, ldc, b, d.sAb, d.sBb,
                            (N + 2) * ldc, flags=L.GEMM_OUT_BF16)
         tb.append((dt, Ct, ref, M, N, keep))
-    ops.StreamGemmGroup([x[0] for x in tb], bm=128, bn=160, device=cuda, a_mn=True).run()
+    ops.StreamGemmGroup([x[0] for x in tb], bm=128, bn=bn, device=cuda, a_mn=True).run()
     torch.cuda.synchronize()
     for (dt, Ct, ref, M, N, _), case in zip(tb, cases):
         got = Ct[:, :N, :M].double().cpu().transpose(1, 2)
