@@ -503,9 +503,11 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
         // fp32 output row-major (Z), bf16 output transposed (T_hat); host-checked
         if (sr.bf16) store_tile_tb16<C>(sr, acc, wm, wn, lane, SO[wave]);
         else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);
-      } else {
+      } else if constexpr ((C::WN_T * 2) % 8 == 0) {
         if (sr.bf16) store_tile<2, C>(sr, acc, wm, wn, lane, SO[wave]);
         else store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);
+      } else {
+        store_tile<4, C>(sr, acc, wm, wn, lane, SO[wave]);   // 128 x 160: fp32 output only (host-checked)
       }
 #pragma unroll
       for (int i = 0; i < WM_T; ++i)
@@ -522,8 +524,8 @@ __global__ __launch_bounds__(C::NT) void gemm_stream_kernel(const FhGemmDesc* __
 }  // namespace
 
 // Streaming bf16 GEMM over a descriptor table prepared by fh_gemm_prepare(descs, n, bm, bn).
-// (bm, bn): (64 | 96 | 128 | 160, 128) with 4 waves and 4 stages, or (128, 256) with 8 waves
-// and 3 stages.  grid: persistent workgroups (0 = one per CU).
+// (bm, bn): (64 | 96 | 128 | 160, 128) and (128, 160) with 4 waves and 4 stages, or (128, 256)
+// with 8 waves and 3 stages.  grid: persistent workgroups (0 = one per CU).
 namespace {
 template <class C, bool F8, bool AMN = false>
 int launch_stream(const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles, int grid, hipStream_t stream) {
@@ -571,6 +573,10 @@ int stream_dispatch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int
     }
   } else if (bn == 256 && bm == 128) {
     return launch_stream<StreamCfg<4, 4, 2, 4, 3>, F8>(descs_dev, n_desc, total_tiles, grid, stream);
+  } else if (bn == 160 && bm == 128) {
+    // 4 waves stacked along m, each over all 160 columns: N = s in (128, 160] without the
+    // 256-wide tile's unused columns
+    return launch_stream<StreamCfg<2, 10, 4, 1, 4>, F8>(descs_dev, n_desc, total_tiles, grid, stream);
   }
   fh_set_error("gemm_stream: unsupported tile %d x %d", bm, bn);
   return FH_ERR_ARG;

@@ -6,7 +6,8 @@ descriptor table launched with one tile shape.  The engine picks shapes by rule 
 covering s, one tile column over s, ...); `tune_engine` instead times every tile shape the
 kernel family has on the engine's own data and keeps the fastest, group by group:
 
-* streaming bf16 / fp8 GEMM: (bm, bn) in (64 | 96 | 128 | 160, 128) and (128, 256) (the
+* streaming bf16 / fp8 GEMM: (bm, bn) in (64 | 96 | 128 | 160, 128), (128, 160) (fp32 output)
+  and (128, 256) (the
   launches of the m-contiguous-A mode keep the width matched to s by rule);
 * tiled fp32 / bf16 MFMA GEMM: the seven shapes 128x128, 128x64, 64x64, 128x160, 64x128,
   256x128, 128x256.
@@ -25,7 +26,7 @@ import torch
 
 from . import ops
 
-STREAM_SHAPES = ((64, 128), (96, 128), (128, 128), (160, 128), (128, 256))
+STREAM_SHAPES = ((64, 128), (96, 128), (128, 128), (160, 128), (128, 160), (128, 256))
 # choices measured on MI355X for the benchmark workloads (read-only default of fit / bench.py)
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_mi355x.json")
 TILED_SHAPES = tuple(range(7))
@@ -76,9 +77,11 @@ def _candidates(g, device):
     if getattr(g, "a_mn", False):
         return None, None                      # width matched to s by rule in this mode
     if isinstance(g, ops.StreamGemmGroup):
+        # the 128 x 160 tile stores fp32 output only
+        bf16_out = any(d.flags & ops.L.GEMM_OUT_BF16 for d in g.descs)
         return [((bm, bn), (lambda bm=bm, bn=bn: ops.StreamGemmGroup(g.descs, bm=bm, bn=bn, grid=g.grid,
                                                                       device=device, fp8=g.fp8)))
-                for bm, bn in STREAM_SHAPES], (g.bm, g.bn)
+                for bm, bn in STREAM_SHAPES if not (bf16_out and bn == 160)], (g.bm, g.bn)
     if isinstance(g, ops.GemmGroup):
         return [(k, (lambda k=k: ops.GemmGroup(g.descs, g.dtype, k, device))) for k in TILED_SHAPES], g.shape
     return None, None

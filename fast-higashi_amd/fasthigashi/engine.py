@@ -509,13 +509,15 @@ class SweepEngine:
             self.g_ct8 = ops.QuantRowsGroup(q, dev)
             narrow = [d for d in d2 if d.N <= 128]
             wide = [d for d in d2 if d.N > 128]
+            wbn = 160 if wide and max(d.N for d in wide) <= 160 else 256
             self.g_y = StreamMulti([ops.StreamGemmGroup(v, bm=128, bn=bn, device=dev, fp8=True)
-                                    for v, bn in ((narrow, 128), (wide, 256)) if v])
+                                    for v, bn in ((narrow, 128), (wide, wbn)) if v])
         elif self.dtype == "bf16":
             narrow = [d for d in d2 if d.N <= 128]
             wide = [d for d in d2 if d.N > 128]
+            wbn = 160 if wide and max(d.N for d in wide) <= 160 else 256
             self.g_y = StreamMulti([ops.StreamGemmGroup(v, bm=128, bn=bn, device=dev)
-                                    for v, bn in ((narrow, 128), (wide, 256)) if v])
+                                    for v, bn in ((narrow, 128), (wide, wbn)) if v])
         else:
             self.g_y = MultiGemm(d2, self.xdt, lambda d: shape_for_n(d.N), dev)
         # (3) Y rows *= A[bin]

@@ -165,7 +165,7 @@ def stream_bm(M: int) -> int:
 
 class StreamGemmGroup:
     """Grouped launch of the streaming bf16 GEMM (both operands k-contiguous, persistent
-    workgroups, global->LDS ring); tiles bm x bn: bm in STREAM_BM with bn = 128, or 128 x 256.
+    workgroups, global->LDS ring); tiles bm x bn: bm in STREAM_BM with bn = 128, or 128 x 160 / 256.
     a_mn: A stored m-contiguous (sAm = 1; read transposed from LDS), 128 x bn tiles (bn in AMN_BN);
     fp32 output (any layout) or bf16 output stored m-contiguous (sCm = 1)."""
 
@@ -210,7 +210,9 @@ class StreamGemmGroup:
             assert not (d.scale_a or d.scale_b), "the streaming GEMM takes no scales (applied outside it)"
         self.bm = bm if bm is not None else stream_bm(max(d.M for d in descs))
         self.bn = bn
-        assert (bn == 128 and self.bm in STREAM_BM) or (bn, self.bm) == (256, 128), "tile %dx%d" % (self.bm, bn)
+        assert (bn == 128 and self.bm in STREAM_BM) or (bn, self.bm) in ((256, 128), (160, 128)), \
+            "tile %dx%d" % (self.bm, bn)
+        assert bn != 160 or all(not (d.flags & L.GEMM_OUT_BF16) for d in descs), "128 x 160 tiles: fp32 output"
         self.grid = grid
         arr = (L.GemmDesc * len(descs))(*descs)
         self.total = L.lib().fh_gemm_prepare(ctypes.cast(arr, ctypes.c_void_p), len(descs), self.bm, bn)

@@ -104,9 +104,9 @@ int fh_gemm64_launch(int a_dtype, int b_dtype, int c_dtype, const FhGemmDesc* de
  * (parafac2_intergrative.py:374-383), T_hat_i = P_i^T X_i (:422-430) and T_tilde_c =
  * T_hat_c M (:522-526).  Both operands k-contiguous, persistent workgroups (grid 0 = one per
  * CU) with a ring of direct global->LDS loads.  descs_dev prepared by fh_gemm_prepare(.., bm,
- * bn) with (bm, bn) one of (64 | 96 | 128 | 160, 128) or (128, 256); leading dims and batch
- * strides multiples of 8, 16-byte aligned bases, rows zero from K to the next multiple of 8
- * when K % 8 != 0; flags 0 or FH_GEMM_OUT_BF16, beta 0. */
+ * bn) with (bm, bn) one of (64 | 96 | 128 | 160, 128), (128, 160) or (128, 256); leading dims
+ * and batch strides multiples of 8, 16-byte aligned bases, rows zero from K to the next multiple
+ * of 8 when K % 8 != 0; flags 0 or FH_GEMM_OUT_BF16 (not with 128 x 160), beta 0. */
 int fh_gemm_stream_launch(int bm, int bn, const FhGemmDesc* descs_dev, int n_desc, int64_t total_tiles,
                           int grid, fh_stream_t stream);
 /* The streaming GEMM with A stored m-contiguous (element (m, k) at A[m + k sAk], sAm = 1) and B

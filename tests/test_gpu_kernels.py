@@ -377,6 +377,26 @@ def test_gemm_stream_m_contiguous_a(cuda, bn):
         assert bool((Ct[:, N:, :].float() == 7.0).all()) and bool((Ct[:, :, M:].float() == 7.0).all()), case
 
 
+def test_gemm_stream_128x160_tile(cuda):
+    """128 x 160 tiles (4 waves stacked along m, fp32 output only): the Y shape for s in
+    (128, 160] (long k, N = s) and ragged cases; bf16 output is refused."""
+    from fasthigashi import ops, _lib as L
+    gen = torch.Generator().manual_seed(41)
+    cases = [(300, 150, 1000, 1, False), (130, 132, 64, 3, False), (77, 160, 130, 2, False), (256, 161, 640, 1, False),
+             (40, 20, 72, 2, False)]
+    built = [_stream_case(cuda, gen, M, N, K, b, o) for (M, N, K, b, o) in cases]
+    ops.StreamGemmGroup([x[0] for x in built], bm=128, bn=160, device=cuda).run()
+    torch.cuda.synchronize()
+    for (d, C, ref, M, _), case in zip(built, cases):
+        got = C[:, :M].double().cpu()
+        err = (got - ref).abs().max().item() / ref.abs().max().item()
+        assert err < 1e-5, (case, err)
+        assert bool((C[:, M:].float() == 7.0).all()), case
+    b16 = _stream_case(cuda, gen, 64, 64, 64, 1, True)
+    with pytest.raises(AssertionError):
+        ops.StreamGemmGroup([b16[0]], bm=128, bn=160, device=cuda)
+
+
 def test_gemm_stream_matches_tiled_gemm(cuda):
     """same descriptors through the streaming kernel and the general tiled bf16 GEMM."""
     from fasthigashi import ops, _lib as L

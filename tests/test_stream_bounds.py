@@ -15,7 +15,8 @@ import pytest
 
 # (bm, bn, m-contiguous A) -> (WM_T, WN_T, WGM, WGN, STAGES), as stream_dispatch picks them
 CFG = {(64, 128, False): (2, 4, 2, 2, 4), (96, 128, False): (3, 4, 2, 2, 4), (128, 128, False): (4, 4, 2, 2, 4),
-       (160, 128, False): (5, 4, 2, 2, 4), (128, 256, False): (4, 4, 2, 4, 3), (128, 64, True): (2, 4, 4, 1, 4),
+       (160, 128, False): (5, 4, 2, 2, 4), (128, 256, False): (4, 4, 2, 4, 3), (128, 160, False): (2, 10, 4, 1, 4),
+       (128, 64, True): (2, 4, 4, 1, 4),
        (128, 96, True): (2, 6, 4, 1, 4), (128, 128, True): (2, 8, 4, 1, 4), (128, 160, True): (2, 10, 4, 1, 4)}
 
 
