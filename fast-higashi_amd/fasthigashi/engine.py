@@ -887,11 +887,11 @@ class SweepEngine:
         if self.f8:
             L.call("fh_scale_rows", self.tt_tab.data_ptr(), self.K, self.tt_max, st)
 
-    def transform(self, stream=None) -> np.ndarray:
-        """Final meta-embedding (parafac2_intergrative.py:742-834): SVD_term from the stored
-        projections and the current A, B, D, then its polar factor.  Leaves self.M unchanged."""
+    def svd_term(self, stream=None) -> torch.Tensor:
+        """SVD_term^T (N x R, fp32, on the device) of parafac2_intergrative.py:742-826 from the
+        stored projections P and the current A, B, D: T_hat = P^T X, Z = T_hat^T L^T, Z D^T.
+        Overwrites the work buffers of a sweep (T_hat, Z, SVD_term), not the factors."""
         st = ops.stream_handle(stream)
-        M_keep = self.M.clone()
         if self.dtype in ("bf16", "fp8"):
             self.g_pt.run(stream)
         if self.f8:
@@ -903,6 +903,14 @@ class SweepEngine:
                    self.pscale.data_ptr() + 4 * cl.psoff[0] if self.f8 else None, st)
         self.g_z.run(stream)
         self.g_svd.run(stream)
+        return self.SVDt[:self.N]
+
+    def transform(self, stream=None) -> np.ndarray:
+        """Final meta-embedding (parafac2_intergrative.py:742-834): SVD_term from the stored
+        projections and the current A, B, D, then its polar factor.  Leaves self.M unchanged."""
+        st = ops.stream_handle(stream)
+        M_keep = self.M.clone()
+        self.svd_term(stream)
         self.g_gm.run(stream)
         L.call("fh_batch_sum", self.GMpart.data_ptr(), F64, self.meta_ksplit, self.R * self.R, self.R * self.R,
                self.Gm.data_ptr(), st)

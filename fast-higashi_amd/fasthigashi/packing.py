@@ -453,8 +453,12 @@ def rwr_padded(Xc: torch.Tensor, w: int, ls: int, epochs: int, do_col: bool = Fa
 
 
 def build_X(engine, coos: Sequence[DeviceCOO], n_i: Sequence[int], do_rwr: bool = True, do_col: bool = False,
-            bin_cov: Optional[Sequence[torch.Tensor]] = None, budget_bytes: int = 8 << 30, do_conv: bool = False):
+            bin_cov: Optional[Sequence[torch.Tensor]] = None, budget_bytes: int = 8 << 30, do_conv: bool = False,
+            cells0: int = 0):
     """Fill engine.X with the post-RWR tiles of every chromosome (fixed epochs n_i[c]).
+
+    Engine cell k is COO cell cells0 + k (cells0 = the good-QC count for an engine over the
+    bad-QC cells, which follow the good ones in the COO); bin_cov rows are engine cells.
 
     X is the same in every sweep (the RWR epochs and bin_cov are fixed after init), so the
     tiles the reference re-densifies and re-imputes on every fetch are built once."""
@@ -465,7 +469,7 @@ def build_X(engine, coos: Sequence[DeviceCOO], n_i: Sequence[int], do_rwr: bool 
             nc_max = tb.chunk_cells(tl.rows, tl.width)
             for k0 in range(0, engine.N, nc_max):
                 cells = slice(k0, min(k0 + nc_max, engine.N))
-                Xc = tb.dense(tl, cells)
+                Xc = tb.dense(tl, slice(cells0 + cells.start, cells0 + cells.stop))
                 if do_conv and tl.rows > 1:                                   # partial_rwr.py:77-81
                     pooled = torch.zeros_like(Xc)
                     wp = Xc.shape[2]

@@ -94,8 +94,6 @@ class Fast_Higashi_core:
         self.B_dict = {c: torch.eye(s).add_(torch.randn(s), alpha=1e-2) for c, s in self.chrom2size.items()}
         bin_cov_list, bad_bin_cov_list, n_i_all, C_blocks = [], [], [], []
         for cd in schic:
-            if cd.num_cell_batch_bad > 0:
-                raise NotImplementedError("bad-QC cells (good_qc_num < total) are not implemented")
             N = cd.num_cell
             bin_cov = torch.full((N, cd.num_bin), 1e-4, dtype=torch.float32, device=dev)
             ll = int(math.ceil(1000000 / cd.resolution))
@@ -136,6 +134,25 @@ class Fast_Higashi_core:
             run_pass(False)
             if do_col:
                 run_pass(True)
+            bad_cov = None
+            if cd.num_cell_batch_bad > 0:                                # :98-99, :179-200
+                # bad-QC cells: coverage only (auto-stop RWR, no column pass, no SVD features)
+                bad_cov = torch.full((cd.total_cell_num - N, cd.num_bin), 1e-4, dtype=torch.float32, device=dev)
+                for tl in cd.tiles:
+                    for cells in cd.cell_slice_list[cd.num_cell_batch:]:
+                        Xc = tb.dense(tl, cells)
+                        if do_conv and tl.rows > 1:
+                            pooled = torch.zeros_like(Xc)
+                            wp = Xc.shape[2]
+                            L.call("fh_avgpool3", Xc.data_ptr(), wp, tl.rows * wp, tl.rows, tl.width, Xc.shape[0],
+                                   pooled.data_ptr(), wp, tl.rows * wp, ops.stream_handle())
+                            Xc = pooled
+                        out = tb.rwr(Xc, tl, -1, False)[0] if do_rwr else Xc
+                        wp = out.shape[2]
+                        cov = bad_cov[cells.start - N: cells.stop - N, tl.col_start: tl.col_start + tl.width]
+                        L.call("fh_colsum_add", out.data_ptr(), wp, tl.rows * wp, tl.rows, tl.width,
+                               out.shape[0], cov.data_ptr(), cd.num_bin, ops.stream_handle())
+                bad_cov[bad_cov <= 1e-4] = float("inf")                   # :270-272
             feats = np.concatenate(blocks, axis=1) if blocks else np.zeros((N, 0), np.float32)
             size = self.chrom2size[cd.chrom]
             svd = TruncatedSVD(n_components=size, n_iter=2, random_state=self.svd_random_state)
@@ -143,7 +160,7 @@ class Fast_Higashi_core:
             n_i_all.append(int(np.max(n_i_list)) if n_i_list else 0)
             bin_cov[bin_cov <= 1e-4] = float("inf")
             bin_cov_list.append(bin_cov)
-            bad_bin_cov_list.append(0)
+            bad_bin_cov_list.append(bad_cov if bad_cov is not None else 0)
         self.n_i = np.array(n_i_all)
         print("rwr iters:", self.n_i)
         C = np.concatenate(C_blocks, axis=1).astype(np.float64)
@@ -248,9 +265,27 @@ class Fast_Higashi_core:
         return self
 
     def transform(self, schic, do_conv, do_rwr, do_col):
-        """Final meta-embedding from the stored projections and factors (:742-834)."""
+        """Final meta-embedding from the stored projections and factors (:742-834).  With bad-QC
+        cells (good_qc_num < total) their post-RWR tiles (epochs n_i, their own bin coverage) are
+        projected with the same P_i, A, B, D and the polar factor is taken over all cells
+        (good first), as the reference's SVD_term over total_cell_num columns."""
         print("start transform")
-        meta_embedding = torch.from_numpy(self.engine.transform())
+        eng = self.engine
+        n_bad = schic[0].total_cell_num - schic[0].num_cell
+        if n_bad == 0:
+            meta_embedding = torch.from_numpy(eng.transform())
+        else:
+            bad = SweepEngine([cd.tile_specs() for cd in schic], [cd.num_bin for cd in schic],
+                              [self.chrom2size[cd.chrom] for cd in schic], n_bad, self.rank,
+                              dtype=self.path_dtype, names=[cd.chrom for cd in schic])
+            build_X(bad, [cd.coo for cd in schic], [int(v) for v in self.n_i], do_rwr=do_rwr, do_col=do_col,
+                    bin_cov=self.bad_bin_cov_list if do_col else None, do_conv=do_conv, cells0=schic[0].num_cell)
+            A, B, D, _ = eng.get_factors()
+            bad.set_factors(A, B, D, np.zeros((n_bad, self.rank), np.float32))
+            bad.P.copy_(eng.P)
+            svd_all = torch.cat([eng.svd_term().clone(), bad.svd_term()])
+            meta_embedding = ops.polar(svd_all)[0].cpu()
+            del bad
         return (None, (self.A_list, self.B_dict.values(), self.D_dict.values(), meta_embedding), self.projection_list)
 
     def fit_transform(self, schic, size_ratio=0.3, n_iter_max=2000, n_iter_parafac=5, do_conv=True, do_rwr=False,

@@ -42,14 +42,14 @@ def test_sparse_sort_matches_oracle(cuda, tiny):
         assert np.array_equal(s.indptr, r["indptr"])
 
 
-def _datasets(wl, raw, bs_bin=50, bs_cell=None, res=None):
+def _datasets(wl, raw, bs_bin=50, bs_cell=None, res=None, good_qc_num=-1):
     from fasthigashi.sparse_for_schic import Chrom_Dataset, Sparse
     out = []
     for c, r in enumerate(raw):
         s = Sparse(r["idx"], r["val"], r["shape"])
         N = r["shape"][2]
         out.append(Chrom_Dataset(s, bs_bin=min(bs_bin, r["shape"][0]), bs_cell=bs_cell or N, compact=True,
-                                 flank=wl.off_diag, chrom="chr%d" % (c + 1),
+                                 flank=wl.off_diag, chrom="chr%d" % (c + 1), good_qc_num=good_qc_num,
                                  resolution=res or wl.resolution))
     return out
 
@@ -170,6 +170,87 @@ def test_core_fit_transform_matches_oracle(cuda, tiny):
                         model.D_dict[cd.chrom].numpy())
         want = np.einsum("ir,jr,kr->ijk", ref.factors.A[c], ref.factors.B[c], ref.factors.D[c])
         assert np.abs(got - want).max() / np.abs(want).max() < 5e-3
+
+
+def test_core_transform_with_bad_qc_cells(cuda, tiny):
+    """good_qc_num < total: the fit runs over the good-QC cells, transform projects every cell
+    (bad-QC ones with the same P_i, A, B, D) and takes the polar factor over all of them
+    (parafac2_intergrative.py:742-834).  Against the oracle's fit over the good cells and its
+    transform over all cells' post-RWR tiles at the fixed epochs n_i."""
+    import dataclasses
+    from fasthigashi.parafac2_intergrative import Fast_Higashi_core
+    from oracle import ref_np as R
+    wl, _, raw, nis = tiny
+    data_all, _, _ = build_dataset(wl, bs_bin=50, n_i=nis)
+    Ntot, Ng = data_all[0].X[0].shape[2], 120
+    data_good = [dataclasses.replace(cd, X=[np.ascontiguousarray(x[:, :, :Ng]) for x in cd.X],
+                                     cell_chunks=[slice(0, Ng)]) for cd in data_all]
+    F0, sizes = random_init(data_good, wl, wl.rank, seed=5)
+    ds = _datasets(wl, raw, bs_cell=64, good_qc_num=Ng)
+    assert ds[0].num_cell == Ng and ds[0].total_cell_num == Ntot and ds[0].num_cell_batch_bad == 1
+    model = Fast_Higashi_core(rank=wl.rank, off_diag=wl.off_diag, res_list=[wl.resolution]).to("cuda")
+    _inject(model, F0, ds, nis)
+    n_it = 5
+    out = model.fit_transform(ds, size_ratio=wl.dim1, n_iter_max=n_it, n_iter_parafac=1, do_conv=False,
+                              do_rwr=True, do_col=False, tol=0, run_init=False, verbose=False)
+    ref = R.fit(data_good, F0, n_it, 1, 0.0, np.float64)
+    rel = np.abs(np.array(model.rec_errors) - np.array(ref.rec_errors)) / np.array(ref.rec_errors)
+    assert rel.max() < 1e-4, rel
+    M = out[1][3].numpy()
+    assert M.shape == (Ntot, wl.rank)
+    assert np.abs(M.T @ M - np.eye(wl.rank)).max() < 1e-4
+    F_all = dataclasses.replace(ref.factors, M=np.zeros((Ntot, wl.rank)))
+    M_ref = R.transform(data_all, F_all, ref.projections, np.float64)
+    cc = canonical_corr(M, M_ref)
+    assert cc.min() > 0.999, cc
+    # the bad-QC rows follow the same linear map as the good ones: Q fitted on the good rows
+    Q = np.linalg.lstsq(M[:Ng].astype(np.float64), M_ref[:Ng], rcond=None)[0]
+    err = np.linalg.norm(M[Ng:] @ Q - M_ref[Ng:]) / np.linalg.norm(M_ref[Ng:])
+    print("bad-QC rows after the good rows' alignment: rel err %.2e, direct %.2e"
+          % (err, np.linalg.norm(M - M_ref) / np.linalg.norm(M_ref)), flush=True)
+    assert err < 2e-2, err
+    assert np.linalg.norm(M - M_ref) / np.linalg.norm(M_ref) < 2e-3      # 2.2e-4 measured
+
+
+def test_core_init_params_bad_qc_coverage(cuda, tiny):
+    """init_params with bad-QC cells: their bin coverage (parafac2_intergrative.py:98-99,
+    179-200, 270-272: auto-stop RWR per bad-cell batch, column sums, <= 1e-4 -> inf); the epoch
+    count n_i and the SVD features come from the good cells alone."""
+    from fasthigashi.parafac2_intergrative import Fast_Higashi_core
+    from oracle import ref_np as R
+    wl, data, raw, nis = tiny
+    Ng = 100
+    ds = _datasets(wl, raw, bs_cell=48, good_qc_num=Ng)
+    model = Fast_Higashi_core(rank=wl.rank, off_diag=wl.off_diag, res_list=[wl.resolution]).to("cuda")
+    model.svd_random_state = 0
+    model._sizes(ds, wl.dim1, None)
+    model.init_params(ds, do_conv=False, do_rwr=True, do_col=True)
+    for c, (cd, r) in enumerate(zip(ds, raw)):
+        Nb = cd.total_cell_num - Ng
+        bad_chunks = cd.cell_slice_list[cd.num_cell_batch:]
+        assert len(bad_chunks) == 2
+        cov = np.full((Nb, cd.num_bin), 1e-4, dtype=np.float64)
+        ni_good = []
+        for tl in cd.tiles:
+            for ch in bad_chunks:
+                coo, v, ps = R.tile_coo(r["sidx"], r["sval"], r["indptr"], r["shape"], tl, ch)
+                x = R.densify(coo, v, ps, transpose=True)
+                out, _ = R.partial_rwr(x, tl.local_start, tl.local_start + tl.rows, False, True, False,
+                                       final_transpose=False)
+                cov[ch.start - Ng: ch.stop - Ng, tl.col_start: tl.col_start + tl.width] += out.sum(1)
+            for ch in cd.cell_slice_list[:cd.num_cell_batch]:
+                coo, v, ps = R.tile_coo(r["sidx"], r["sval"], r["indptr"], r["shape"], tl, ch)
+                x = R.densify(coo, v, ps, transpose=True)
+                ni_good.append(R.partial_rwr(x, tl.local_start, tl.local_start + tl.rows, False, True, False,
+                                             final_transpose=False)[1])
+        assert model.n_i[c] == max(ni_good)
+        cov[cov <= 1e-4] = np.inf
+        got = model.bad_bin_cov_list[c].cpu().numpy()
+        fin = np.isfinite(cov)
+        assert got.shape == cov.shape and np.array_equal(np.isfinite(got), fin)
+        assert np.abs(got[fin] - cov[fin]).max() / cov[fin].max() < 1e-5
+        assert model.bin_cov_list[c].shape == (Ng, cd.num_bin)
+    assert model.meta_embedding.shape == (Ng, wl.rank)
 
 
 def test_core_init_params(cuda, tiny):
